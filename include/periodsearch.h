@@ -1,0 +1,185 @@
+/*
+ * periodsearch.h -- C ABI of the MI355X period-search engine (libpsamd.so).
+ *
+ * The engine replaces the trial-period x initial-pole Levenberg-Marquardt fit
+ * loop of the Asteroids@home period_search app.  Its only backend seam in the
+ * reference is the C++ API of the CUDA app,
+ *   period_search_cuda/period_search/start_CUDA.h:2-14
+ *   (CUDAPrepare / CUDAPrecalc / CUDAStart / CUDAUnprepare),
+ * which reads most of its inputs from extern globals
+ * (period_search_cuda/period_search/globals.h:6-20).  This header exposes the
+ * same roles as plain C functions over caller-owned buffers, with no globals:
+ *
+ *   ps_create + ps_set_problem   <- CUDAPrepare (start_CUDA.cu:119) and the
+ *                                   static uploads of CUDAStart (start_CUDA.cu:558-647)
+ *   ps_precalc                   <- CUDAPrecalc (start_CUDA.cu:248), one conw round
+ *   ps_scan                      <- CUDAStart (start_CUDA.cu:523) without the text
+ *                                   sink: it returns FP64 rows, the host writer
+ *                                   (ps_format_row, ps_write_checkpoint) owns the text
+ *   ps_destroy                   <- CUDAUnprepare (start_CUDA.cu:240)
+ *   ps_fit_batch, ps_eval_mrqcof -- test seams with no reference counterpart
+ *                                   (per-fit results, one normal-equation build)
+ *
+ * Host-side helpers replace the driver code that surrounds those calls in
+ * period_search/period_search_BOINC.cpp (input parsing :277-485, shape
+ * setup :737-782, row text :926-931, checkpoint :85-101).
+ *
+ * Conventions: arrays are 0-indexed and dense; the engine copies what it
+ * needs, the caller keeps ownership.  Return value 0 = OK, negative = error
+ * (ps_last_error gives the text).  One engine handle per device; handles are
+ * independent, calls on one handle must not overlap.
+ */
+#ifndef PERIODSEARCH_H
+#define PERIODSEARCH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS_OK 0
+#define PS_ERR_ARG -1
+#define PS_ERR_HIP -2
+#define PS_ERR_LIMIT -3
+#define PS_ERR_IO -4
+#define PS_ERR_STATE -5
+
+/* Arithmetic of the model/Jacobian stage.  PS_PREC_FP64 is the plain form
+ * (everything FP64); PS_PREC_FP32 runs the facet stage and the
+ * brightness/derivative contraction in FP32 (phase reduction, normal
+ * equations, solve, chi2 and acceptance stay FP64). */
+#define PS_PREC_FP64 0
+#define PS_PREC_FP32 1
+
+typedef struct ps_engine ps_engine;
+
+/* One asteroid: points, shape basis and fit constants. */
+typedef struct ps_problem {
+  /* observations (the engine appends the 3 convexity rows itself) */
+  int n_points;              /* real points, = ndata before the conv rows */
+  int n_lc;                  /* real light curves */
+  const int *lc_points;      /* [n_lc] points per light curve */
+  const int *lc_inrel;       /* [n_lc] 1 = relative light curve (Inrel = 1 - flag) */
+  const double *tim;         /* [n_points] JD - jd0 [days] */
+  const double *brightness;  /* [n_points] */
+  const double *sig;         /* [n_points] per-light-curve mean brightness */
+  const double *weight;      /* [n_points] per-point weight */
+  const double *ee;          /* [n_points*3] unit vector asteroid -> Earth */
+  const double *ee0;         /* [n_points*3] unit vector asteroid -> Sun */
+  /* shape basis on the Gaussian sphere (period_search/globals.h:5-19) */
+  int lmax, mmax;
+  int n_fac;                 /* F = 8*nrows^2 */
+  const double *nor;         /* [F*3] facet normals */
+  const double *darea;       /* [F] facet areas on the unit sphere */
+  const double *pleg;        /* [F*(lmax+1)*(lmax+1)] P_l^m, index (f*(lmax+1)+l)*(lmax+1)+m */
+  const double *fc;          /* [F*(mmax+1)] cos(m*phi) */
+  const double *fs;          /* [F*(mmax+1)] sin(m*phi) */
+  const double *dsph;        /* [F*ncoef] */
+  const double *cg_first;    /* [ncoef] initial ellipsoid shape */
+  /* fit constants (period_search_in header) */
+  double phi0;               /* initial rotation angle [rad] */
+  double par[3];             /* phase-function a, d, k */
+  int ia_par[3];
+  double cl;                 /* Lambert weight (fitted as log cl) */
+  int ia_cl;
+  int ia_prd;                /* period free */
+  int onlyrel;               /* 1 if every light curve is relative (fixes shape coef 1) */
+  double alamda_start, alamda_incr;
+  double stop_condition;     /* >1: that many iterations; <1: dev threshold */
+  int n_iter_min;
+  int n_poles;               /* initial poles, normally 10 */
+  const double *beta_pole;   /* [n_poles] deg */
+  const double *lambda_pole; /* [n_poles] deg */
+} ps_problem;
+
+/* Result of one fit (one trial period from one initial pole). */
+typedef struct ps_fit_out {
+  double dev;      /* rms: sqrt(rchisq/(ndata-3)) */
+  double chisq;    /* final Chisq */
+  double rchisq;   /* Chisq minus the convexity penalty */
+  double prd;      /* fitted period [days] */
+  double la;       /* lambda [deg] (not wrapped) */
+  double be;       /* beta [deg] */
+  double dark;     /* |sum Area*Nor| */
+  double totarea;  /* sum of Area of the last shape evaluation */
+  double alamda;   /* final Marquardt lambda */
+  int niter;       /* LM iterations run */
+  int err;         /* last gauss_errc code (0 = none) */
+} ps_fit_out;
+
+/* One output row: the six numbers of period_search_out. */
+typedef struct ps_row {
+  double period_h;  /* 24 * per_best */
+  double rms;       /* dev_best */
+  double chisq;     /* dev_best^2 * (ndata-3) */
+  double dark;      /* dark-facet % (row n=1 prints conw_r*escl^2 instead) */
+  double lambda;    /* deg, wrapped to [0,360) */
+  double beta;      /* deg */
+} ps_row;
+
+/* ---- engine ---------------------------------------------------------- */
+int ps_create(int device, ps_engine **out);
+int ps_destroy(ps_engine *e);
+const char *ps_last_error(const ps_engine *e);
+/* Select the arithmetic of the model stage (PS_PREC_*). Default FP64. */
+int ps_set_precision(ps_engine *e, int precision);
+
+int ps_set_problem(ps_engine *e, const ps_problem *p);
+
+/* Upper bound on concurrently resident fits (slots).  Default 2048.  Each
+ * slot holds one fit's LM state and Jacobian in HBM. */
+int ps_set_slots(ps_engine *e, int max_slots);
+
+/* One convexity-weight round (CUDAPrecalc): fits min(10, nPeriods) periods
+ * x n_poles at conw_r, averages the best dark-facet %, sets *new_conw = 1 if
+ * the average is < 1, else doubles *conw_r.  Frequencies are 1/P with P
+ * in hours, as in the reference driver (period_search_BOINC.cpp:784-786). */
+int ps_precalc(ps_engine *e, double freq_start, double freq_end, double freq_step, double *conw_r,
+               int *new_conw);
+
+/* Rows n_from..n_to (1-based, inclusive) of the scan at conw_r (CUDAStart).
+ * rows: [n_to-n_from+1].  ndata_minus_3 is taken from the problem. */
+int ps_scan(ps_engine *e, double freq_start, double freq_step, int n_from, int n_to, double conw_r,
+            ps_row *rows);
+
+/* Test seam: n independent fits; freq [n] (1/h), pole [n] (0-based index
+ * into the problem's pole table). */
+int ps_fit_batch(ps_engine *e, const double *freq, const int *pole, int n, double conw_r, ps_fit_out *out);
+
+/* Test seam: one normal-equation build (mrqcof) at each of n parameter
+ * vectors cg [n*ma] (ma = ncoef+8, reference order).  alpha [n*mfit*mfit]
+ * (full symmetric), beta [n*mfit], chisq [n]; chck [n*3] = sum Area*Nor. */
+int ps_eval_mrqcof(ps_engine *e, const double *cg, int n, double conw_r, double *alpha, double *beta,
+                   double *chisq, double *chck);
+
+/* Problem dimensions after ps_set_problem: ncoef, ma, mfit. */
+int ps_dims(const ps_engine *e, int *ncoef, int *ma, int *mfit);
+
+/* Time of the last fit-kernel launch [ms] measured with HIP events on the
+ * engine's stream, and the number of fits it ran. */
+int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits);
+
+/* ---- host-side driver helpers ---------------------------------------- */
+/* Parsed and normalised period_search_in (period_search_BOINC.cpp:277-485). */
+typedef struct ps_input ps_input;
+int ps_input_read(const char *path, ps_input **out);
+void ps_input_free(ps_input *in);
+/* Fill a ps_problem that points into the input's storage (valid until
+ * ps_input_free).  Builds the shape basis (trifac, areanorm, sphfunc, ellfit). */
+int ps_input_problem(ps_input *in, ps_problem *p);
+/* Scalars of the input: per_start, per_end, per_step_coef, jd_min, jd_max,
+ * escl, conw, freq_start, freq_end, freq_step; ndata (real points). */
+int ps_input_scalars(const ps_input *in, double *out10, int *ndata);
+/* period_search_out row text, "%.8f  %.6f  %.6f %4.1f %4.0f %4.0f\n";
+ * round_pole != 0 prints round(lambda), round(beta) as the CUDA/OC apps do. */
+int ps_format_row(const ps_row *r, int round_pole, char *buf, int buflen);
+
+/* Shape basis alone: nrows, lmax, mmax, ellipsoid axes -> tables sized as in
+ * ps_problem (nor [F*3], darea [F], pleg [F*(lmax+1)^2], fc/fs [F*(mmax+1)],
+ * dsph [F*ncoef], cg_first [ncoef]).  Returns F, or <0. */
+int ps_shape_basis(int nrows, int lmax, int mmax, double a, double b, double c, double *nor, double *darea,
+                   double *pleg, double *fc, double *fs, double *dsph, double *cg_first);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

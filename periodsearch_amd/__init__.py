@@ -1,0 +1,277 @@
+"""MI355X period-search engine: Python mirror of the C ABI in include/periodsearch.h.
+
+The compute path is the HIP library ``libpsamd.so`` built in this directory
+(``python -m periodsearch_amd.build``).  Importing this package loads it; if it
+is missing or cannot be loaded the import fails loudly: there is no CPU
+fallback for the fit loop.
+
+The classes follow the roles of the reference's CUDA backend
+(period_search_cuda/period_search/start_CUDA.h:2-14):
+
+    Engine(device)            CUDAPrepare's device/context part
+    Engine.set_problem(p)     CUDAPrepare + the static uploads of CUDAStart
+    Engine.precalc(...)       CUDAPrecalc (one convexity-weight round)
+    Engine.scan(...)          CUDAStart (rows as numbers; text via format_row)
+    Engine.fit_batch(...)     per-fit results (test seam)
+    Engine.eval_mrqcof(...)   one normal-equation build (test seam)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsamd.so")
+DRIVER_PATH = os.path.join(HERE, "period_search_amd")
+
+PS_PREC_FP64 = 0
+PS_PREC_FP32 = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class PsProblem(C.Structure):
+    _fields_ = [
+        ("n_points", C.c_int), ("n_lc", C.c_int), ("lc_points", _ip), ("lc_inrel", _ip),
+        ("tim", _dp), ("brightness", _dp), ("sig", _dp), ("weight", _dp), ("ee", _dp), ("ee0", _dp),
+        ("lmax", C.c_int), ("mmax", C.c_int), ("n_fac", C.c_int),
+        ("nor", _dp), ("darea", _dp), ("pleg", _dp), ("fc", _dp), ("fs", _dp), ("dsph", _dp), ("cg_first", _dp),
+        ("phi0", C.c_double), ("par", C.c_double * 3), ("ia_par", C.c_int * 3), ("cl", C.c_double),
+        ("ia_cl", C.c_int), ("ia_prd", C.c_int), ("onlyrel", C.c_int),
+        ("alamda_start", C.c_double), ("alamda_incr", C.c_double), ("stop_condition", C.c_double),
+        ("n_iter_min", C.c_int), ("n_poles", C.c_int), ("beta_pole", _dp), ("lambda_pole", _dp),
+    ]
+
+
+class PsFitOut(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dev", "chisq", "rchisq", "prd", "la", "be", "dark", "totarea", "alamda")] + [
+        ("niter", C.c_int), ("err", C.c_int)]
+
+
+class PsRow(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("period_h", "rms", "chisq", "dark", "lambda_", "beta")]
+
+
+FIT_DTYPE = np.dtype([(n, "f8") for n in ("dev", "chisq", "rchisq", "prd", "la", "be", "dark", "totarea", "alamda")]
+                     + [("niter", "i4"), ("err", "i4")])
+ROW_DTYPE = np.dtype([(n, "f8") for n in ("period_h", "rms", "chisq", "dark", "lambda", "beta")])
+assert FIT_DTYPE.itemsize == C.sizeof(PsFitOut)
+assert ROW_DTYPE.itemsize == C.sizeof(PsRow)
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"periodsearch_amd: {LIB_PATH} is not built; run `python -m periodsearch_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "ps_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "ps_destroy": (C.c_int, [vp]),
+        "ps_last_error": (C.c_char_p, [vp]),
+        "ps_set_precision": (C.c_int, [vp, C.c_int]),
+        "ps_set_slots": (C.c_int, [vp, C.c_int]),
+        "ps_set_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
+        "ps_precalc": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, _dp, _ip]),
+        "ps_scan": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, vp]),
+        "ps_fit_batch": (C.c_int, [vp, _dp, _ip, C.c_int, C.c_double, vp]),
+        "ps_eval_mrqcof": (C.c_int, [vp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
+        "ps_dims": (C.c_int, [vp, _ip, _ip, _ip]),
+        "ps_last_kernel_ms": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
+        "ps_input_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "ps_input_free": (None, [vp]),
+        "ps_input_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
+        "ps_input_scalars": (C.c_int, [vp, _dp, _ip]),
+        "ps_format_row": (C.c_int, [C.POINTER(PsRow), C.c_int, C.c_char_p, C.c_int]),
+        "ps_shape_basis": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+# Symbols declared in include/periodsearch.h (checked by the CPU test suite).
+EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_set_slots", "ps_set_problem",
+            "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
+            "ps_input_read", "ps_input_free", "ps_input_problem", "ps_input_scalars", "ps_format_row",
+            "ps_shape_basis")
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+class PsError(RuntimeError):
+    pass
+
+
+def n_coef(lmax: int, mmax: int) -> int:
+    return sum(2 if m else 1 for m in range(mmax + 1) for _ in range(m, lmax + 1))
+
+
+def shape_basis(nrows: int, lmax: int, mmax: int, a: float, b: float, c: float) -> dict:
+    """Gaussian-sphere facet tables and ellipsoid coefficients (host code)."""
+    F = 8 * nrows * nrows
+    nc = n_coef(lmax, mmax)
+    out = dict(nor=np.zeros(F * 3), darea=np.zeros(F), pleg=np.zeros(F * (lmax + 1) ** 2),
+               fc=np.zeros(F * (mmax + 1)), fs=np.zeros(F * (mmax + 1)), dsph=np.zeros(F * nc),
+               cg_first=np.zeros(nc))
+    rc = lib.ps_shape_basis(nrows, lmax, mmax, a, b, c, *(_d(out[k]) for k in
+                                                          ("nor", "darea", "pleg", "fc", "fs", "dsph", "cg_first")))
+    if rc < 0:
+        raise PsError(f"ps_shape_basis failed: {rc}")
+    out["nor"] = out["nor"].reshape(F, 3)
+    out["dsph"] = out["dsph"].reshape(F, nc)
+    out["pleg"] = out["pleg"].reshape(F, lmax + 1, lmax + 1)
+    out["fc"] = out["fc"].reshape(F, mmax + 1)
+    out["fs"] = out["fs"].reshape(F, mmax + 1)
+    return out
+
+
+class Input:
+    """A parsed period_search_in (period_search/period_search_BOINC.cpp:277-485)."""
+
+    def __init__(self, path: str):
+        h = C.c_void_p()
+        rc = lib.ps_input_read(os.fsencode(path), C.byref(h))
+        if rc != 0:
+            raise PsError(f"cannot read {path}: {rc}")
+        self._h = h
+        self.problem = PsProblem()
+        lib.ps_input_problem(self._h, C.byref(self.problem))
+        s = np.zeros(10)
+        nd = C.c_int()
+        lib.ps_input_scalars(self._h, _d(s), C.byref(nd))
+        (self.per_start, self.per_end, self.per_step_coef, self.jd_min, self.jd_max, self.escl, self.conw,
+         self.freq_start, self.freq_end, self.freq_step) = s.tolist()
+        self.ndata = nd.value
+        self.n_periods = int((self.freq_start - self.freq_end) / self.freq_step) + 1
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.ps_input_free(h)
+            self._h = None
+
+    @property
+    def conw_r0(self) -> float:
+        return self.conw / self.escl / self.escl
+
+
+class Engine:
+    """One engine handle on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib.ps_create(device, C.byref(h))
+        if rc != 0:
+            raise PsError(f"ps_create({device}) failed: {rc} (no HIP device?)")
+        self._h = h
+        self._keep = None
+
+    def close(self):
+        if self._h:
+            lib.ps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise PsError(f"{what} failed ({rc}): {lib.ps_last_error(self._h).decode()}")
+
+    def set_precision(self, prec: int):
+        self._check(lib.ps_set_precision(self._h, prec), "ps_set_precision")
+
+    def set_slots(self, n: int):
+        self._check(lib.ps_set_slots(self._h, n), "ps_set_slots")
+
+    def set_problem(self, problem: PsProblem, keep=None):
+        self._keep = keep  # keep the arrays behind the pointers alive
+        self._check(lib.ps_set_problem(self._h, C.byref(problem)), "ps_set_problem")
+
+    def dims(self):
+        nc, ma, mf = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib.ps_dims(self._h, C.byref(nc), C.byref(ma), C.byref(mf)), "ps_dims")
+        return nc.value, ma.value, mf.value
+
+    def fit_batch(self, freq, pole, conw_r: float) -> np.ndarray:
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        pole = np.ascontiguousarray(pole, dtype=np.int32)
+        out = np.zeros(len(freq), dtype=FIT_DTYPE)
+        self._check(lib.ps_fit_batch(self._h, _d(freq), _i(pole), len(freq), conw_r, out.ctypes.data),
+                    "ps_fit_batch")
+        return out
+
+    def scan(self, freq_start: float, freq_step: float, n_from: int, n_to: int, conw_r: float) -> np.ndarray:
+        rows = np.zeros(n_to - n_from + 1, dtype=ROW_DTYPE)
+        self._check(lib.ps_scan(self._h, freq_start, freq_step, n_from, n_to, conw_r, rows.ctypes.data), "ps_scan")
+        return rows
+
+    def precalc(self, freq_start: float, freq_end: float, freq_step: float, conw_r: float):
+        cw = C.c_double(conw_r)
+        nc = C.c_int(0)
+        self._check(lib.ps_precalc(self._h, freq_start, freq_end, freq_step, C.byref(cw), C.byref(nc)),
+                    "ps_precalc")
+        return cw.value, nc.value
+
+    def eval_mrqcof(self, cg: np.ndarray, conw_r: float):
+        cg = np.ascontiguousarray(np.atleast_2d(cg), dtype=np.float64)
+        n = cg.shape[0]
+        _, ma, mfit = self.dims()
+        assert cg.shape[1] == ma
+        alpha = np.zeros((n, mfit, mfit))
+        beta = np.zeros((n, mfit))
+        chi = np.zeros(n)
+        chck = np.zeros((n, 3))
+        self._check(lib.ps_eval_mrqcof(self._h, _d(cg), n, conw_r, _d(alpha), _d(beta), _d(chi), _d(chck)),
+                    "ps_eval_mrqcof")
+        return alpha, beta, chi, chck
+
+    def last_kernel_ms(self):
+        ms = C.c_double()
+        n = C.c_longlong()
+        lib.ps_last_kernel_ms(self._h, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+
+def format_row(row, round_pole: bool = False) -> str:
+    r = PsRow(*(float(row[k]) for k in ROW_DTYPE.names))
+    buf = C.create_string_buffer(256)
+    lib.ps_format_row(C.byref(r), 1 if round_pole else 0, buf, 256)
+    return buf.value.decode()
+
+
+def run_scan(inp: Input, eng: Engine, n_from: int = 1, n_to: int | None = None, conw_r: float | None = None,
+             batch: int = 512) -> tuple[list[str], float]:
+    """Driver loop of period_search_BOINC.cpp:487-931 on the engine: conw
+    precalc (unless conw_r is given), then rows n_from..n_to as text."""
+    if n_to is None:
+        n_to = inp.n_periods
+    if conw_r is None:
+        conw_r = inp.conw_r0
+        new_conw = 0
+        while new_conw != 1 and conw_r * inp.escl * inp.escl < 10.0:
+            conw_r, new_conw = eng.precalc(inp.freq_start, inp.freq_end, inp.freq_step, conw_r)
+    lines = []
+    for n0 in range(n_from, n_to + 1, batch):
+        n1 = min(n_to, n0 + batch - 1)
+        rows = eng.scan(inp.freq_start, inp.freq_step, n0, n1, conw_r)
+        for k, r in enumerate(rows):
+            r = r.copy()
+            if n0 + k == 1:
+                r["dark"] = conw_r * inp.escl * inp.escl
+            lines.append(format_row(r))
+    return lines, conw_r

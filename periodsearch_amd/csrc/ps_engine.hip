@@ -1,0 +1,1475 @@
+// ps_engine.hip -- MI355X (gfx950) engine for the trial-period x initial-pole
+// Levenberg-Marquardt fit loop of period_search (Asteroids@home).
+//
+// Design: continuous batching over persistent fit slots.  A fixed set of
+// slots lives in HBM; each slot holds the complete LM state of one fit
+// (parameters, damped/accepted normal matrices, the Jacobian of the last
+// model evaluation).  One LM iteration of every active slot is three
+// kernels, each sized for its own work and register budget:
+//
+//   step_kernel   one workgroup per slot: finishes a converged fit and takes
+//                 the next fit from the work queue, or damps the accepted
+//                 normal equations and solves them in LDS (Gauss-Jordan with
+//                 full pivoting, period_search/gauss_errc.c:9-73) to form
+//                 the trial parameters (period_search/mrqmin.c:53-79).
+//   model_kernel  one workgroup per (slot, tile of 256 points): curv, the
+//                 per-point rotation/phase geometry in FP64 and the facet
+//                 loop with the brightness-derivative contraction
+//                 (period_search/bright.c, curv.c, matrix.c, phasec.c).
+//                 Facet normals and the Dsph basis are read-only tables
+//                 shared by every fit, read through the scalar cache.
+//   neq_kernel    one workgroup per slot: convexity rows (conv.c), relative
+//                 renormalisation and the normal equations (mrqcof.c), then
+//                 accept/reject (mrqmin.c:82-107) and the driver bookkeeping
+//                 (period_search/period_search_BOINC.cpp:868-892).
+//
+// A slot whose fit ends is refilled from the queue in the next step_kernel,
+// so fits with different iteration counts keep every slot busy.  A last
+// kernel reduces the poles of each trial period to one output row
+// (period_search_BOINC.cpp:913-924).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "periodsearch.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxNcoef = 128;
+constexpr int kMaxMa = kMaxNcoef + 8;
+constexpr int kMaxMfit = kMaxMa;
+constexpr int kChunk = 32;         // points staged per normal-equation chunk
+constexpr int kMaxFacets = 1000;
+constexpr double kTiny = 1e-8;     // period_search/constants.h:10
+constexpr double kPi = 3.14159265358979323846;
+
+#define HIP_TRY(x)                                                         \
+  do {                                                                     \
+    hipError_t err_ = (x);                                                 \
+    if (err_ != hipSuccess) {                                              \
+      e->err = std::string(#x) + ": " + hipGetErrorString(err_);           \
+      return PS_ERR_HIP;                                                   \
+    }                                                                      \
+  } while (0)
+
+// Read-only problem tables in device memory plus the constants of the fit.
+struct DevProblem {
+  // points, SoA, real points only
+  const double *tim, *y, *sig, *w, *ex, *ey, *ez, *sx, *sy, *sz;
+  const int *lc_start;  // [n_lc + 2]: real light curves then the conv "light curve"
+  const int *lc_inrel;  // [n_lc + 1]
+  int n_points, n_lc;
+  // facets
+  const double *nx, *ny, *nz, *darea;
+  const double *dsph;     // [F][ncp], zero-padded columns
+  const double *cpleg;    // [ngrp][F]: P_l^m of coefficient group q at facet f
+  const double *cfc;      // [mmax+1][F]
+  const double *cfs;      // [mmax+1][F]
+  const int *grp_m;       // [ngrp]
+  int nfac, ngrp, ncoef, ncp, ma, mfit;
+  const int *fidx;        // [mfit] free parameter -> index into a[]
+  const double *cg_first; // [ncoef]
+  const double *pole_beta, *pole_lambda;  // [n_poles] deg
+  double phi0, alamda_start, alamda_incr, log_cl, par0, par1, par2;
+  int n_iter_max, n_iter_min;
+  double iter_diff_max;
+  int ndata;   // n_points + 3
+  int npad;    // Jacobian leading dimension (>= n_points + 3)
+  int ntiles;  // point tiles of kThreads
+};
+
+// Per-slot LM state (scalars).  Vectors and matrices live in the slot's
+// block of slot memory (SlotMem).
+struct SlotState {
+  int item, phase, niter, err;
+  int eval, sel, pad0, pad1;
+  double lam, ochisq, chisq, chitr, dev_new, dev_old, iter_diff, rchisq;
+  double chck[3], chtr[3];
+};
+enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_DRAINED = 3 };
+
+// Offsets (in doubles) inside one slot's memory block.
+struct SlotMem {
+  long long J, alpha0, alpha1, beta0, beta1, a, atry, alast, total;
+};
+
+__host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
+  SlotMem m;
+  long long o = 0;
+  m.J = o; o += (long long)(ma + 1) * npad;
+  m.alpha0 = o; o += (long long)mfit * mfit;
+  m.alpha1 = o; o += (long long)mfit * mfit;
+  m.beta0 = o; o += kMaxMfit;
+  m.beta1 = o; o += kMaxMfit;
+  m.a = o; o += kMaxMa;
+  m.atry = o; o += kMaxMa;
+  m.alast = o; o += kMaxMa;
+  o = (o + 31) / 32 * 32;  // 256-byte aligned blocks
+  m.total = o;
+  return m;
+}
+
+// One batch of work: fits (mode 0) or single model evaluations (mode 1).
+struct DevWork {
+  int n_items, mode, nslots, pad;
+  const double *freq;       // [n_items] (mode 0)
+  const int *pole;          // [n_items] (mode 0)
+  const double *cg_in;      // [n_items][ma] (mode 1)
+  double conw_r;
+  ps_fit_out *out;          // [n_items] (mode 0)
+  double *ev_alpha, *ev_beta, *ev_chi, *ev_chck;  // (mode 1)
+  int *queue, *done;
+  SlotState *st;
+  double *mem;
+  long long slot_doubles;
+};
+
+// curv (period_search/curv.c:10-38): Area and g of every facet for the
+// shape part of a.  Same operation order as the reference.
+__device__ void eval_curv(const DevProblem &P, const double *a, double *area, double *g) {
+#pragma clang fp contract(off)
+  for (int f = threadIdx.x; f < P.nfac; f += kThreads) {
+    double s = 0;
+    int n = 0;
+    for (int q = 0; q < P.ngrp; q++) {
+      int m = P.grp_m[q];
+      double fsum = a[n] * P.cfc[m * P.nfac + f];
+      n++;
+      if (m != 0) {
+        fsum = fsum + a[n] * P.cfs[m * P.nfac + f];
+        n++;
+      }
+      s = s + P.cpleg[q * P.nfac + f] * fsum;
+    }
+    double gf = exp(s);
+    g[f] = gf;
+    area[f] = P.darea[f] * gf;
+  }
+}
+
+// blmatrix (period_search/blmatrix.c:9-46) into LDS: B[9], dB/dbeta[9], dB/dlambda[9].
+__device__ void eval_blmatrix(double bet, double lam, double *blm) {
+  double cb = cos(bet), sb = sin(bet), cl = cos(lam), sl = sin(lam);
+  double *B = blm, *D1 = blm + 9, *D2 = blm + 18;
+  B[0] = cb * cl; B[1] = cb * sl; B[2] = -sb;
+  B[3] = -sl;     B[4] = cl;      B[5] = 0;
+  B[6] = sb * cl; B[7] = sb * sl; B[8] = cb;
+  D1[0] = -sb * cl; D1[1] = -sb * sl; D1[2] = -cb;
+  D1[3] = 0;        D1[4] = 0;        D1[5] = 0;
+  D1[6] = cb * cl;  D1[7] = cb * sl;  D1[8] = -sb;
+  D2[0] = -cb * sl; D2[1] = cb * cl;  D2[2] = 0;
+  D2[3] = -cl;      D2[4] = -sl;      D2[5] = 0;
+  D2[6] = -sb * sl; D2[7] = sb * cl;  D2[8] = 0;
+}
+
+// Per-point geometry, part 1: the phase function (period_search/phasec.c:11-24)
+// and the rotated Earth/Sun directions e = tmat ee, e0 = tmat ee0 with
+// tmat = Rz(f) Blmat, f = fmod(omega t + phi0, 2 pi) (period_search/matrix.c:11-58,
+// bright.c:36-55).  Same operations and order as the reference.
+struct PointGeom {
+  double e[3], e0[3];
+  double cf, sf, t;
+  double scale, dphp[3];
+};
+
+__device__ void point_geometry(const DevProblem &P, int p, const double *a, const double *blm, PointGeom &G) {
+#pragma clang fp contract(off)
+  const int nc = P.ncoef;
+  const double ee[3] = {P.ex[p], P.ey[p], P.ez[p]};
+  const double e0[3] = {P.sx[p], P.sy[p], P.sz[p]};
+  const double t = P.tim[p];
+  double cos_alpha = ee[0] * e0[0] + ee[1] * e0[1] + ee[2] * e0[2];
+  double alpha = acos(cos_alpha);
+  double p1 = a[nc + 3], p2 = a[nc + 4], p3 = a[nc + 5];
+  double ex = exp(-alpha / p2);
+  G.scale = 1 + p1 * ex + p3 * alpha;
+  G.dphp[0] = ex;
+  G.dphp[1] = p1 * ex * alpha / (p2 * p2);
+  G.dphp[2] = alpha;
+  double f = a[nc + 2] * t + P.phi0;
+  f = fmod(f, 2 * kPi);
+  const double cf = cos(f), sf = sin(f);
+  const double *B = blm;
+  for (int i = 0; i < 3; i++) {
+    // row i of tmat = fmat row i times Blmat
+    double t0, t1, t2;
+    if (i == 0) {
+      t0 = cf * B[0] + sf * B[3];
+      t1 = cf * B[1] + sf * B[4];
+      t2 = cf * B[2] + sf * B[5];
+    } else if (i == 1) {
+      t0 = (-sf) * B[0] + cf * B[3];
+      t1 = (-sf) * B[1] + cf * B[4];
+      t2 = (-sf) * B[2] + cf * B[5];
+    } else {
+      t0 = B[6];
+      t1 = B[7];
+      t2 = B[8];
+    }
+    G.e[i] = t0 * ee[0] + t1 * ee[1] + t2 * ee[2];
+    G.e0[i] = t0 * e0[0] + t1 * e0[1] + t2 * e0[2];
+  }
+  G.cf = cf;
+  G.sf = sf;
+  G.t = t;
+}
+
+// Per-point geometry, part 2: derivatives of e and e0 with respect to
+// (beta, lambda, omega) contracted with the facet sums V = sum Area dsmu Nor
+// and U = sum Area dsmu0 Nor: r_k = sum_i de_i,k V_i + de0_i,k U_i
+// (bright.c:41-55 and :88-103 regrouped).
+__device__ void rotation_derivs(const DevProblem &P, int p, const double *blm, const PointGeom &G, const double V[3],
+                                const double U[3], double r[3]) {
+  const double ee[3] = {P.ex[p], P.ey[p], P.ez[p]};
+  const double e0[3] = {P.sx[p], P.sy[p], P.sz[p]};
+  const double cf = G.cf, sf = G.sf, t = G.t;
+  for (int k = 0; k < 3; k++) {
+    // dtm_k rows 0..2
+    double m[3][3];
+    if (k < 2) {
+      const double *D = blm + 9 * (k + 1);
+      for (int j = 0; j < 3; j++) {
+        m[0][j] = cf * D[j] + sf * D[3 + j];
+        m[1][j] = (-sf) * D[j] + cf * D[3 + j];
+        m[2][j] = D[6 + j];
+      }
+    } else {
+      const double *B = blm;
+      for (int j = 0; j < 3; j++) {
+        m[0][j] = ((-t) * sf) * B[j] + (t * cf) * B[3 + j];
+        m[1][j] = ((-t) * cf) * B[j] + ((-t) * sf) * B[3 + j];
+        m[2][j] = 0;
+      }
+    }
+    double s = 0;
+    for (int i = 0; i < 3; i++) {
+      const double de = m[i][0] * ee[0] + m[i][1] * ee[1] + m[i][2] * ee[2];
+      const double de0 = m[i][0] * e0[0] + m[i][1] * e0[1] + m[i][2] * e0[2];
+      s += de * V[i] + de0 * U[i];
+    }
+    r[k] = s;
+  }
+}
+
+// Pass 1 for one tile of kThreads points: brightness and its derivatives
+// (period_search/bright.c:14-126), rows written to the slot scratch J
+// (column-major, leading dimension npad; row ma holds ymod).
+// NCB = accumulator count (ncoef rounded up to 8; Dsph is zero-padded).
+template <int NCB>
+__device__ void model_tile(const DevProblem &P, int p0, const double *a, const double *blm, const double *area,
+                           double *J) {
+  const int lane_p = p0 + (int)threadIdx.x;
+  const int wave_first = p0 + ((int)threadIdx.x & ~63);
+  if (wave_first >= P.n_points) return;  // whole wave idle
+  const int p = lane_p < P.n_points ? lane_p : P.n_points - 1;
+  PointGeom G;
+  point_geometry(P, p, a, blm, G);
+  const double cl = exp(a[P.ma - 2]);
+  const double cls = a[P.ma - 1];
+  double S[NCB];
+#pragma unroll
+  for (int c = 0; c < NCB; c++) S[c] = 0;
+  double br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
+  const double e_0 = G.e[0], e_1 = G.e[1], e_2 = G.e[2];
+  const double s_0 = G.e0[0], s_1 = G.e0[1], s_2 = G.e0[2];
+  const double *__restrict__ nxp = P.nx;
+  const double *__restrict__ nyp = P.ny;
+  const double *__restrict__ nzp = P.nz;
+  const double *__restrict__ dsp = P.dsph;
+#pragma unroll 1
+  for (int f = 0; f < P.nfac; f++) {
+    const double n0 = nxp[f], n1 = nyp[f], n2 = nzp[f];
+    const double mu = e_0 * n0 + e_1 * n1 + e_2 * n2;
+    const double mu0 = s_0 * n0 + s_1 * n1 + s_2 * n2;
+    if (mu > kTiny && mu0 > kTiny) {
+      const double af = area[f];
+      const double rd = 1.0 / (mu + mu0);
+      const double mm = mu * mu0;
+      const double s = mm * (cl + cls * rd);
+      const double wgt = af * s;
+      br += wgt;
+      const double q0 = mu0 * rd, q1 = mu * rd;
+      const double ad = af * (cls * q0 * q0 + cl * mu0);
+      const double ad0 = af * (cls * q1 * q1 + cl * mu);
+      v0 += ad * n0;
+      v1 += ad * n1;
+      v2 += ad * n2;
+      u0 += ad0 * n0;
+      u1 += ad0 * n1;
+      u2 += ad0 * n2;
+      const double am = af * mm;
+      scl += am;
+      scls += am * rd;
+      const double *drow = dsp + (size_t)f * NCB;
+#pragma unroll
+      for (int c = 0; c < NCB; c++) S[c] += wgt * drow[c];
+    }
+  }
+  if (lane_p >= P.n_points) return;
+  const int ld = P.npad;
+  const double sc = G.scale;
+#pragma unroll
+  for (int c = 0; c < NCB; c++)
+    if (c < P.ncoef) J[(size_t)c * ld + p] = sc * S[c];
+  const double V[3] = {v0, v1, v2}, U[3] = {u0, u1, u2};
+  double r[3];
+  rotation_derivs(P, p, blm, G, V, U, r);
+  for (int k = 0; k < 3; k++) J[(size_t)(P.ncoef + k) * ld + p] = sc * r[k];
+  for (int i = 0; i < 3; i++) J[(size_t)(P.ncoef + 3 + i) * ld + p] = br * G.dphp[i];
+  J[(size_t)(P.ncoef + 6) * ld + p] = sc * scl * cl;
+  J[(size_t)(P.ncoef + 7) * ld + p] = sc * scls;
+  J[(size_t)P.ma * ld + p] = br * sc;
+}
+
+// The three convexity rows (period_search/conv.c:12-29) at points
+// n_points..n_points+2, and chck = sum Area*Nor.  Serial over facets in the
+// reference order.
+__device__ void conv_rows(const DevProblem &P, const double *area, const double *g, double *J, double *chck) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, ld = P.npad, n = P.n_points;
+  if (tid < P.ncoef) {
+    double r0 = 0, r1 = 0, r2 = 0;
+    for (int f = 0; f < P.nfac; f++) {
+      double dg = g[f] * P.dsph[(size_t)f * P.ncp + tid];
+      double t = P.darea[f] * dg;
+      r0 += t * P.nx[f];
+      r1 += t * P.ny[f];
+      r2 += t * P.nz[f];
+    }
+    J[(size_t)tid * ld + n] = r0;
+    J[(size_t)tid * ld + n + 1] = r1;
+    J[(size_t)tid * ld + n + 2] = r2;
+  } else if (tid < P.ma) {
+    J[(size_t)tid * ld + n] = 0;
+    J[(size_t)tid * ld + n + 1] = 0;
+    J[(size_t)tid * ld + n + 2] = 0;
+  } else if (tid >= kThreads - 3) {
+    const int nc = tid - (kThreads - 3);
+    const double *nv = nc == 0 ? P.nx : (nc == 1 ? P.ny : P.nz);
+    double r = 0;
+    for (int f = 0; f < P.nfac; f++) r += area[f] * nv[f];
+    J[(size_t)P.ma * ld + n + nc] = r;
+    chck[nc] = r;
+  }
+}
+
+// Lower-triangle entry e -> (j, k), j >= k, e = j(j+1)/2 + k.
+__device__ inline void tri_index(int e, int &j, int &k) {
+  int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= e) r++;
+  while (r * (r + 1) / 2 > e) r--;
+  j = r;
+  k = e - r * (r + 1) / 2;
+}
+
+// Pass 2: relative renormalisation and the normal equations
+// (period_search/mrqcof.c:42-139).  Points are staged kChunk at a time in
+// LDS as renormalised rows Jn and weighted rows WT = Jn * sig2i * w; each
+// lower-triangle entry of alpha is owned by one thread and summed serially
+// over points in the reference's order, accumulating in the per-slot alpha
+// buffer (global, L2-resident).  beta and chi2 likewise.  alpha is
+// mirrored to full symmetric storage at the end.
+__device__ void normal_equations(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
+                                 double *beta, double *chi_out, double *alpha) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, ld = P.npad, mfit = P.mfit, ma = P.ma;
+  const int nent = mfit * (mfit + 1) / 2;
+  double bacc = 0, chi = 0;
+  bool first = true;
+  double *Jn = wst;                          // [kChunk][mfit]
+  double *WT = wst + kChunk * mfit;          // [kChunk][mfit]
+  double *aux = wst + 2 * kChunk * mfit;     // [kChunk][3]: sig2i, weight, dy
+  for (int lc = 0; lc <= P.n_lc; lc++) {
+    const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
+    const int inrel = P.lc_inrel[lc];
+    if (inrel) {
+      // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda
+      if (tid <= ma) {
+        const double *col = J + (size_t)tid * ld;
+        double s = 0;
+        for (int q = q0; q < q1; q++) s = s + col[q];
+        dave[tid] = s;
+      }
+      __syncthreads();
+    }
+    for (int c0 = q0; c0 < q1; c0 += kChunk) {
+      const int cn = min(kChunk, q1 - c0);
+      for (int pp = tid; pp < cn; pp += kThreads) {
+        const int q = c0 + pp;
+        double ymod = J[(size_t)ma * ld + q];
+        double sg, wg, yo;
+        if (q < P.n_points) {
+          sg = P.sig[q];
+          wg = P.w[q];
+          yo = P.y[q];
+        } else {
+          sg = 1 / conw_r;
+          wg = 1;
+          yo = 0;
+        }
+        if (inrel) {
+          const double ave = dave[ma];
+          const double coef = sg * np / ave;
+          ymod = coef * ymod;
+        }
+        aux[pp * 3 + 0] = 1 / (sg * sg);
+        aux[pp * 3 + 1] = wg;
+        aux[pp * 3 + 2] = yo - ymod;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < cn * mfit; idx += kThreads) {
+        const int pp = idx / mfit, jf = idx - pp * mfit;
+        const int q = c0 + pp;
+        const int l = P.fidx[jf];
+        double d = J[(size_t)l * ld + q];
+        if (inrel) {
+          const double ave = dave[ma];
+          const double coef = (q < P.n_points ? P.sig[q] : 1 / conw_r) * np / ave;
+          d = (l == 0) ? 0.0 : coef * (d - J[(size_t)ma * ld + q] * dave[l] / ave);
+        }
+        Jn[pp * mfit + jf] = d;
+        WT[pp * mfit + jf] = d * aux[pp * 3] * aux[pp * 3 + 1];
+      }
+      __syncthreads();
+      for (int e = tid; e < nent; e += kThreads) {
+        int j, k;
+        tri_index(e, j, k);
+        double s = first ? 0.0 : alpha[j * mfit + k];
+        for (int pp = 0; pp < cn; pp++) s = s + WT[pp * mfit + j] * Jn[pp * mfit + k];
+        alpha[j * mfit + k] = s;
+      }
+      if (tid < mfit) {
+        for (int pp = 0; pp < cn; pp++) bacc = bacc + aux[pp * 3 + 2] * WT[pp * mfit + tid];
+      }
+      if (tid == kThreads - 1) {
+        for (int pp = 0; pp < cn; pp++) {
+          const double dy = aux[pp * 3 + 2];
+          chi = chi + dy * dy * aux[pp * 3] * aux[pp * 3 + 1];
+        }
+      }
+      first = false;
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < nent; e += kThreads) {
+    int j, k;
+    tri_index(e, j, k);
+    if (j != k) alpha[k * mfit + j] = alpha[j * mfit + k];
+  }
+  if (tid < mfit) beta[tid] = bacc;
+  if (tid == kThreads - 1) *chi_out = chi;
+}
+
+// Gauss-Jordan elimination with full pivoting on the LDS matrix A (n x n)
+// and right-hand side b (period_search/gauss_errc.c:9-73).  Only the
+// solution b is needed by the caller, so columns that are already pivoted
+// are not updated; every value that the reference's b depends on is
+// computed with the reference's operations in the reference's order.
+// Returns 0, 1 (singular) or 2 (zero pivot), uniform across the workgroup.
+__device__ int gauss_solve(double *A, int n, double *b, int *ipiv, double *prow, double *dumv,
+                                        double *red) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = tid; j < n; j += kThreads) ipiv[j] = 0;
+  __syncthreads();
+  int prev_irow = -1, prev_icol = -1;
+  for (int i = 0; i < n; i++) {
+    // 1. argmax |A[j][k]| over unpivoted rows/columns; ties keep the last
+    //    entry in row-major order (fabs(a) >= big in the reference).
+    double bv = -1.0;
+    int be = -1;
+    for (int e = tid; e < n * n; e += kThreads) {
+      const int j = e / n, k = e - j * n;
+      if (ipiv[j] != 1 && ipiv[k] == 0) {
+        const double v = fabs(A[e]);
+        if (v >= bv) {
+          bv = v;
+          be = e;
+        }
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      double ov = __shfl_xor(bv, o, 64);
+      int oe = __shfl_xor(be, o, 64);
+      if (ov > bv || (ov == bv && oe > be)) {
+        bv = ov;
+        be = oe;
+      }
+    }
+    if (lane == 0) {
+      red[2 * wave] = bv;
+      red[2 * wave + 1] = (double)be;
+    }
+    __syncthreads();
+    bv = red[0];
+    be = (int)red[1];
+    for (int w = 1; w < kWaves; w++) {
+      double ov = red[2 * w];
+      int oe = (int)red[2 * w + 1];
+      if (ov > bv || (ov == bv && oe > be)) {
+        bv = ov;
+        be = oe;
+      }
+    }
+    int irow, icol;
+    if (be < 0) {  // no candidate (NaN matrix): the reference keeps the previous pivot
+      if (prev_icol < 0) return 1;
+      irow = prev_irow;
+      icol = prev_icol;
+    } else {
+      irow = be / n;
+      icol = be - irow * n;
+    }
+    if (ipiv[icol] != 0) return 1;  // column pivoted twice: the next scan reports singular
+    prev_irow = irow;
+    prev_icol = icol;
+    // 2. swap rows irow and icol in the columns that are still live
+    if (irow != icol) {
+      for (int l = tid; l < n; l += kThreads) {
+        if (ipiv[l] == 0) {
+          double t = A[irow * n + l];
+          A[irow * n + l] = A[icol * n + l];
+          A[icol * n + l] = t;
+        }
+      }
+      if (tid == 0) {
+        double t = b[irow];
+        b[irow] = b[icol];
+        b[icol] = t;
+      }
+    }
+    __syncthreads();
+    // 3. pivot row scaled by 1/pivot, multipliers of the other rows
+    const double piv = A[icol * n + icol];
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    for (int l = tid; l < n; l += kThreads) {
+      prow[l] = (ipiv[l] == 0 && l != icol) ? A[icol * n + l] * pivinv : 0.0;
+      dumv[l] = A[l * n + icol];
+    }
+    __syncthreads();
+    const double bpiv = b[icol] * pivinv;
+    // 4. eliminate
+    for (int e = tid; e < n * n; e += kThreads) {
+      const int ll = e / n, l = e - ll * n;
+      if (ipiv[l] == 0 && l != icol) {
+        if (ll == icol)
+          A[e] = prow[l];
+        else
+          A[e] = A[e] - prow[l] * dumv[ll];
+      }
+    }
+    for (int ll = tid; ll < n; ll += kThreads) {
+      if (ll != icol) b[ll] = b[ll] - bpiv * dumv[ll];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      b[icol] = bpiv;
+      ipiv[icol] = 1;
+    }
+    __syncthreads();
+  }
+  return 0;
+}
+
+// Fit initialisation (period_search_BOINC.cpp:819-844): shape from the
+// ellipsoid, pole m, omega from the trial frequency, phase and scattering
+// parameters.
+__device__ void init_params(const DevProblem &P, double freq, int m, double *a) {
+#pragma clang fp contract(off)
+  const int nc = P.ncoef;
+  for (int l = threadIdx.x; l < P.ma; l += kThreads) {
+    double v;
+    if (l < nc)
+      v = P.cg_first[l];
+    else if (l == nc)
+      v = (kPi / 180) * (90 - P.pole_beta[m]);
+    else if (l == nc + 1)
+      v = (kPi / 180) * P.pole_lambda[m];
+    else if (l == nc + 2)
+      v = 24 * 2 * kPi / (1 / freq);
+    else if (l == nc + 3)
+      v = P.par0;
+    else if (l == nc + 4)
+      v = P.par1;
+    else if (l == nc + 5)
+      v = P.par2;
+    else if (l == nc + 6)
+      v = P.log_cl;
+    else
+      v = 1;
+    a[l] = v;
+  }
+}
+
+// ---------------------------------------------------------------------
+// step_kernel: one workgroup per slot.
+// LDS: solve matrix [mfit*mfit], da, prow, dumv [mfit], reduction [8],
+// area/g [F] (for the final curv of a finished fit), ipiv [mfit] ints.
+__global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W) {
+  extern __shared__ double sm[];
+  const int tid = threadIdx.x, slot = blockIdx.x;
+  const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
+  SlotState &S = W.st[slot];
+  const SlotMem M = slot_mem(ma, mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  double *A = sm, *da = A + mfit * mfit, *prow = da + kMaxMfit, *dumv = prow + kMaxMfit, *red = dumv + kMaxMfit;
+  double *area = red + 2 * kWaves, *g = area + P.nfac;
+  int *ipiv = reinterpret_cast<int *>(g + P.nfac);
+  __shared__ int s_phase, s_go, s_item;
+
+  if (tid == 0) {
+    int ph = S.phase;
+    int go = 0;
+    if (ph == PH_ITER) {
+      // loop predicate (period_search_BOINC.cpp:868)
+      go = (((S.niter < P.n_iter_max) && (S.iter_diff > P.iter_diff_max)) || (S.niter < P.n_iter_min)) ? 1 : 0;
+    }
+    s_phase = ph;
+    s_go = go;
+    s_item = S.item;
+  }
+  __syncthreads();
+  int ph = s_phase;
+  if (ph == PH_DRAINED || ph == PH_INIT) {
+    if (tid == 0) S.eval = (ph == PH_INIT) ? 1 : 0;
+    return;
+  }
+  if (ph == PH_ITER && !s_go) {
+    // per-fit summary (period_search_BOINC.cpp:900-911); totarea from the
+    // shape of the last curv() call.
+    eval_curv(P, mem + M.alast, area, g);
+    __syncthreads();
+    if (tid == 0) {
+#pragma clang fp contract(off)
+      double tot = 0;
+      for (int f = 0; f < P.nfac; f++) tot = tot + area[f];
+      const double *a = mem + M.a;
+      const double c0 = S.chck[0], c1 = S.chck[1], c2 = S.chck[2];
+      ps_fit_out o;
+      o.dev = S.dev_new;
+      o.chisq = S.chisq;
+      o.rchisq = S.rchisq;
+      o.prd = 2 * kPi / a[nc + 2];
+      o.la = (180 / kPi) * a[nc + 1];
+      o.be = 90 - (180 / kPi) * a[nc];
+      o.dark = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+      o.totarea = tot;
+      o.alamda = S.lam;
+      o.niter = S.niter;
+      o.err = S.err;
+      W.out[s_item] = o;
+      atomicAdd(W.done, 1);
+    }
+    ph = PH_EMPTY;
+  }
+  if (ph == PH_EMPTY) {
+    if (tid == 0) {
+      int item = atomicAdd(W.queue, 1);
+      s_item = item;
+      s_phase = item < W.n_items ? PH_INIT : PH_DRAINED;
+    }
+    __syncthreads();
+    const int item = s_item;
+    if (s_phase == PH_DRAINED) {
+      if (tid == 0) {
+        S.phase = PH_DRAINED;
+        S.eval = 0;
+      }
+      return;
+    }
+    double *a = mem + M.a, *atry = mem + M.atry;
+    if (W.mode == 1) {
+      for (int l = tid; l < ma; l += kThreads) a[l] = W.cg_in[(size_t)item * ma + l];
+    } else {
+      init_params(P, W.freq[item], W.pole[item], a);
+    }
+    for (int l = tid; l < ma; l += kThreads) atry[l] = a[l];
+    if (tid == 0) {
+      S.item = item;
+      S.phase = PH_INIT;
+      S.eval = 1;
+      S.err = 0;
+    }
+    return;
+  }
+  // damped normal equations and solve (mrqmin.c:53-79)
+  const double *aacc = mem + (S.sel ? M.alpha1 : M.alpha0);
+  const double *bacc = mem + (S.sel ? M.beta1 : M.beta0);
+  {
+#pragma clang fp contract(off)
+    const double lam = S.lam;
+    for (int e = tid; e < mfit * mfit; e += kThreads) {
+      const int j = e / mfit, k = e - j * mfit;
+      A[e] = (j == k) ? aacc[e] * (1 + lam) : aacc[e];
+    }
+    for (int j = tid; j < mfit; j += kThreads) da[j] = bacc[j];
+  }
+  __syncthreads();
+  const int err = gauss_solve(A, mfit, da, ipiv, prow, dumv, red);
+  __syncthreads();
+  double *a = mem + M.a, *atry = mem + M.atry;
+  if (err == 0) {
+#pragma clang fp contract(off)
+    for (int l = tid; l < ma; l += kThreads) atry[l] = a[l];
+    __syncthreads();
+    for (int jf = tid; jf < mfit; jf += kThreads) {
+      const int l = P.fidx[jf];
+      atry[l] = a[l] + da[jf];
+    }
+  }
+  if (tid == 0) {
+    S.eval = err == 0 ? 1 : 0;
+    S.err = err;
+  }
+}
+
+// model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
+template <int NCB>
+__global__ void __launch_bounds__(kThreads) model_kernel(DevProblem P, DevWork W) {
+  extern __shared__ double sm[];
+  const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
+  if (!W.st[slot].eval) return;
+  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  double *x = sm, *blm = x + kMaxMa, *area = blm + 32, *g = area + P.nfac;
+  for (int l = threadIdx.x; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
+  __syncthreads();
+  eval_curv(P, x, area, g);
+  if (threadIdx.x == 0) eval_blmatrix(x[P.ncoef], x[P.ncoef + 1], blm);
+  __syncthreads();
+  model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J);
+}
+
+// neq_kernel: one workgroup per slot.  Convexity rows, normal equations at
+// the trial parameters, then accept/reject and the driver bookkeeping.
+// LDS: staging [2*kChunk*mfit + 3*kChunk], area/g [F], dave [ma+1], x [ma],
+// scalars.
+__global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) {
+  extern __shared__ double sm[];
+  const int tid = threadIdx.x, slot = blockIdx.x;
+  SlotState &S = W.st[slot];
+  const int ph = S.phase;
+  if (ph != PH_INIT && ph != PH_ITER) return;
+  const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
+  const SlotMem M = slot_mem(ma, mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  double *wst = sm, *area = wst + 2 * kChunk * mfit + 3 * kChunk, *g = area + P.nfac, *dave = g + P.nfac;
+  double *x = dave + kMaxMa + 1, *scal = x + kMaxMa;  // scal[0] = chi2, [1..3] = chck
+  const int eval = S.eval;
+  const int sel = S.sel;
+  double *atr = mem + (sel ? M.alpha0 : M.alpha1), *btr = mem + (sel ? M.beta0 : M.beta1);
+  if (eval) {
+    for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
+    __syncthreads();
+    eval_curv(P, x, area, g);
+    __syncthreads();
+    conv_rows(P, area, g, mem + M.J, scal + 1);
+    __threadfence_block();
+    __syncthreads();
+    normal_equations(P, mem + M.J, W.conw_r, wst, dave, btr, scal, atr);
+    __threadfence_block();
+    __syncthreads();
+  }
+  const double conw_r = W.conw_r;
+  double *a = mem + M.a, *atry = mem + M.atry, *alast = mem + M.alast;
+  if (ph == PH_INIT) {
+    if (W.mode == 1) {
+      const int item = S.item;
+      for (int e = tid; e < mfit * mfit; e += kThreads) W.ev_alpha[(size_t)item * mfit * mfit + e] = atr[e];
+      for (int j = tid; j < mfit; j += kThreads) W.ev_beta[(size_t)item * mfit + j] = btr[j];
+      if (tid == 0) {
+        W.ev_chi[item] = scal[0];
+        for (int k = 0; k < 3; k++) W.ev_chck[(size_t)item * 3 + k] = scal[1 + k];
+        S.phase = PH_EMPTY;
+        S.eval = 0;
+        atomicAdd(W.done, 1);
+      }
+      return;
+    }
+    // first mrqmin call (mrqmin.c:36-51) and the driver's start values
+    // (period_search_BOINC.cpp:860-866)
+    for (int l = tid; l < nc; l += kThreads) alast[l] = a[l];
+    if (tid == 0) {
+      S.sel = 1 - sel;
+      S.lam = P.alamda_start;
+      S.ochisq = scal[0];
+      S.chisq = scal[0];
+      for (int k = 0; k < 3; k++) S.chck[k] = scal[1 + k];
+      S.dev_old = 1e30;
+      S.dev_new = 0;
+      S.iter_diff = 1e40;
+      S.rchisq = -1;
+      S.niter = 0;
+      S.err = 0;
+      S.phase = PH_ITER;
+      S.eval = 0;
+    }
+    return;
+  }
+  // accept or reject the trial (mrqmin.c:82-107)
+  __shared__ int s_accept;
+  if (eval) {
+    for (int l = tid; l < nc; l += kThreads) alast[l] = x[l];
+    if (tid == 0) s_accept = scal[0] < S.ochisq ? 1 : 0;
+    __syncthreads();
+    if (s_accept)
+      for (int l = tid; l < ma; l += kThreads) a[l] = x[l];
+    if (tid == 0) {
+#pragma clang fp contract(off)
+      S.chisq = scal[0];
+      if (s_accept) {
+        S.lam = S.lam / P.alamda_incr;
+        S.sel = 1 - sel;
+        for (int k = 0; k < 3; k++) S.chck[k] = scal[1 + k];
+      } else {
+        S.lam = P.alamda_incr * S.lam;
+        S.chisq = S.ochisq;
+      }
+    }
+    __syncthreads();
+  }
+  // driver bookkeeping (period_search_BOINC.cpp:868-892)
+  __shared__ int s_improved;
+  if (tid == 0) {
+    S.niter = S.niter + 1;
+    s_improved = ((S.niter == 1) || (S.chisq < S.ochisq)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (s_improved)
+    for (int l = tid; l < nc; l += kThreads) alast[l] = a[l];
+  if (tid == 0) {
+#pragma clang fp contract(off)
+    if (s_improved) {
+      S.ochisq = S.chisq;
+      const double c0 = S.chck[0], c1 = S.chck[1], c2 = S.chck[2];
+      S.rchisq = S.chisq - (c0 * c0 + c1 * c1 + c2 * c2) * (conw_r * conw_r);
+    }
+    const double dev_new = sqrt(S.rchisq / (P.ndata - 3));
+    S.dev_new = dev_new;
+    if (S.dev_old - dev_new > 1e-10) {
+      S.iter_diff = S.dev_old - dev_new;
+      S.dev_old = dev_new;
+    }
+    S.eval = 0;
+  }
+  (void)atry;
+}
+
+__global__ void reset_batch(SlotState *st, int nslots, int *queue, int *done) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) {
+    SlotState s{};
+    s.phase = PH_EMPTY;
+    s.item = -1;
+    st[i] = s;
+  }
+  if (i == 0) {
+    *queue = 0;
+    *done = 0;
+  }
+}
+
+// Best pole per trial period (period_search_BOINC.cpp:913-924): the first
+// pole with the strictly smallest dev wins; lambda wrapped to [0,360).
+__global__ void reduce_poles(const ps_fit_out *fits, int n_periods, int n_poles, int ndata, int precalc,
+                             ps_row *rows) {
+#pragma clang fp contract(off)
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_periods) return;
+  double dev_best = 1e40, per_best = 0, dark_best = 0, la_best = 0, be_best = 0;
+  for (int m = 0; m < n_poles; m++) {
+    const ps_fit_out &o = fits[(size_t)n * n_poles + m];
+    if (o.dev < dev_best) {
+      dev_best = o.dev;
+      per_best = o.prd;
+      dark_best = o.dark / o.totarea * 100;
+      la_best = o.la;
+      be_best = o.be;
+    }
+  }
+  if (la_best < 0) la_best += 360;
+  if (precalc && isnan(dark_best)) dark_best = 1.0;
+  ps_row r;
+  r.period_h = 24 * per_best;
+  r.rms = dev_best;
+  r.chisq = dev_best * dev_best * (ndata - 3);
+  r.dark = dark_best;
+  r.lambda = la_best;
+  r.beta = be_best;
+  rows[n] = r;
+}
+
+// Trial frequencies and poles of periods n_from.. (period_search_BOINC.cpp:812, 817).
+__global__ void make_items(double freq_start, double freq_step, int n_from, int n_periods, int n_poles,
+                           double *freq, int *pole) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_periods * n_poles) return;
+  const int n = n_from + i / n_poles;
+  freq[i] = freq_start - (n - 1) * freq_step;
+  pole[i] = i % n_poles;
+}
+
+using ModelFn = void (*)(DevProblem, DevWork);
+
+ModelFn pick_model(int ncp) {
+  switch (ncp) {
+    case 8: return model_kernel<8>;
+    case 16: return model_kernel<16>;
+    case 24: return model_kernel<24>;
+    case 32: return model_kernel<32>;
+    case 40: return model_kernel<40>;
+    case 48: return model_kernel<48>;
+    case 56: return model_kernel<56>;
+    case 64: return model_kernel<64>;
+    case 72: return model_kernel<72>;
+    case 80: return model_kernel<80>;
+    case 88: return model_kernel<88>;
+    case 96: return model_kernel<96>;
+    case 104: return model_kernel<104>;
+    case 112: return model_kernel<112>;
+    case 120: return model_kernel<120>;
+    case 128: return model_kernel<128>;
+    default: return nullptr;
+  }
+}
+
+size_t step_lds(int mfit, int nfac) {
+  return ((size_t)mfit * mfit + 3 * kMaxMfit + 2 * kWaves + 2 * (size_t)nfac + kMaxMfit / 2 + 2) * sizeof(double);
+}
+size_t model_lds(int nfac) { return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double); }
+size_t neq_lds(int mfit, int nfac) {
+  return ((size_t)2 * kChunk * mfit + 3 * kChunk + 2 * (size_t)nfac + 2 * kMaxMa + 1 + 8) * sizeof(double);
+}
+
+}  // namespace
+
+struct ps_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  int precision = PS_PREC_FP64;
+  bool have_problem = false;
+  DevProblem P{};
+  std::vector<void *> prob_allocs;
+  int n_poles = 0, cu_count = 0, max_slots = 2048;
+  size_t lds_step = 0, lds_model = 0, lds_neq = 0;
+  ModelFn model = nullptr;
+  // slot memory
+  SlotState *st = nullptr;
+  double *mem = nullptr;
+  size_t mem_slots = 0;
+  long long slot_doubles = 0;
+  int *counters = nullptr;      // [0] queue, [1] done
+  int *host_done = nullptr;     // pinned
+  // item buffers
+  double *d_freq = nullptr;
+  int *d_pole = nullptr;
+  ps_fit_out *d_out = nullptr;
+  ps_row *d_rows = nullptr;
+  size_t item_cap = 0, row_cap = 0;
+  double *d_cg = nullptr, *d_ea = nullptr, *d_eb = nullptr, *d_ec = nullptr, *d_ek = nullptr;
+  size_t ev_cap = 0;
+  double last_ms = 0;
+  long long last_items = 0, last_iters = 0;
+};
+
+namespace {
+
+template <typename T>
+int upload(ps_engine *e, const T *src, size_t n, const T **dst) {
+  void *p = nullptr;
+  HIP_TRY(hipMalloc(&p, n * sizeof(T) + 16));
+  e->prob_allocs.push_back(p);
+  if (n) HIP_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  *dst = static_cast<const T *>(p);
+  return PS_OK;
+}
+
+void free_problem(ps_engine *e) {
+  for (void *p : e->prob_allocs) (void)hipFree(p);
+  e->prob_allocs.clear();
+  e->have_problem = false;
+}
+
+int ensure_items(ps_engine *e, size_t n) {
+  if (n <= e->item_cap) return PS_OK;
+  if (e->d_freq) (void)hipFree(e->d_freq);
+  if (e->d_pole) (void)hipFree(e->d_pole);
+  if (e->d_out) (void)hipFree(e->d_out);
+  e->d_freq = nullptr;
+  e->d_pole = nullptr;
+  e->d_out = nullptr;
+  e->item_cap = 0;
+  size_t cap = n < 1024 ? 1024 : n;
+  HIP_TRY(hipMalloc(&e->d_freq, cap * sizeof(double)));
+  HIP_TRY(hipMalloc(&e->d_pole, cap * sizeof(int)));
+  HIP_TRY(hipMalloc(&e->d_out, cap * sizeof(ps_fit_out)));
+  e->item_cap = cap;
+  return PS_OK;
+}
+
+int ensure_rows(ps_engine *e, size_t n) {
+  if (n <= e->row_cap) return PS_OK;
+  if (e->d_rows) (void)hipFree(e->d_rows);
+  e->d_rows = nullptr;
+  e->row_cap = 0;
+  size_t cap = n < 256 ? 256 : n;
+  HIP_TRY(hipMalloc(&e->d_rows, cap * sizeof(ps_row)));
+  e->row_cap = cap;
+  return PS_OK;
+}
+
+int ensure_slots(ps_engine *e, int nslots) {
+  const SlotMem M = slot_mem(e->P.ma, e->P.mfit, e->P.npad);
+  if ((size_t)nslots <= e->mem_slots && M.total == e->slot_doubles) return PS_OK;
+  if (e->mem) (void)hipFree(e->mem);
+  if (e->st) (void)hipFree(e->st);
+  e->mem = nullptr;
+  e->st = nullptr;
+  e->mem_slots = 0;
+  HIP_TRY(hipMalloc(&e->mem, (size_t)nslots * M.total * sizeof(double)));
+  HIP_TRY(hipMalloc(&e->st, (size_t)nslots * sizeof(SlotState)));
+  e->mem_slots = nslots;
+  e->slot_doubles = M.total;
+  return PS_OK;
+}
+
+// Run W.n_items items through the slots until all are done.
+int run_batch(ps_engine *e, DevWork W) {
+  if (W.n_items <= 0) return PS_OK;
+  int nslots = e->max_slots < W.n_items ? e->max_slots : W.n_items;
+  int rc;
+  if ((rc = ensure_slots(e, nslots))) return rc;
+  W.nslots = nslots;
+  W.queue = e->counters;
+  W.done = e->counters + 1;
+  W.st = e->st;
+  W.mem = e->mem;
+  W.slot_doubles = e->slot_doubles;
+  const DevProblem &P = e->P;
+  hipLaunchKernelGGL(reset_batch, dim3((nslots + 255) / 256), dim3(256), 0, e->stream, e->st, nslots, W.queue,
+                     W.done);
+  HIP_TRY(hipGetLastError());
+  const int per_fit = (P.n_iter_max > P.n_iter_min ? P.n_iter_max : P.n_iter_min) + 3;
+  const long long rounds = (W.n_items + nslots - 1) / nslots + 1;
+  const long long guard = rounds * per_fit * 2 + 64;
+  const int poll = 8;
+  HIP_TRY(hipEventRecord(e->ev0, e->stream));
+  long long it = 0;
+  while (true) {
+    for (int k = 0; k < poll; k++) {
+      hipLaunchKernelGGL(step_kernel, dim3(nslots), dim3(kThreads), e->lds_step, e->stream, P, W);
+      hipLaunchKernelGGL(e->model, dim3(nslots * P.ntiles), dim3(kThreads), e->lds_model, e->stream, P, W);
+      hipLaunchKernelGGL(neq_kernel, dim3(nslots), dim3(kThreads), e->lds_neq, e->stream, P, W);
+    }
+    HIP_TRY(hipGetLastError());
+    it += poll;
+    HIP_TRY(hipMemcpyAsync(e->host_done, W.done, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (*e->host_done >= W.n_items) break;
+    if (it > guard) {
+      e->err = "fit batch did not finish within its iteration bound";
+      return PS_ERR_STATE;
+    }
+  }
+  HIP_TRY(hipEventRecord(e->ev1, e->stream));
+  HIP_TRY(hipEventSynchronize(e->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  e->last_ms = ms;
+  e->last_items = W.n_items;
+  e->last_iters = it;
+  return PS_OK;
+}
+
+int check_ready(ps_engine *e) {
+  if (!e) return PS_ERR_ARG;
+  if (!e->have_problem) {
+    e->err = "no problem set (ps_set_problem)";
+    return PS_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(e->device));
+  return PS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps_create(int device, ps_engine **out) {
+  if (!out) return PS_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PS_ERR_HIP;
+  auto *e = new ps_engine();
+  e->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    delete e;
+    return PS_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete e;
+    return PS_ERR_HIP;
+  }
+  e->cu_count = prop.multiProcessorCount;
+  *out = e;
+  return PS_OK;
+}
+
+int ps_destroy(ps_engine *e) {
+  if (!e) return PS_ERR_ARG;
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->stream);
+  free_problem(e);
+  void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows,
+                  e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (e->host_done) (void)hipHostFree(e->host_done);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return PS_OK;
+}
+
+const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int ps_set_precision(ps_engine *e, int precision) {
+  if (!e) return PS_ERR_ARG;
+  if (precision != PS_PREC_FP64) {
+    e->err = "only PS_PREC_FP64 is available in this build";
+    return PS_ERR_ARG;
+  }
+  e->precision = precision;
+  return PS_OK;
+}
+
+int ps_set_slots(ps_engine *e, int max_slots) {
+  if (!e || max_slots < 1) return PS_ERR_ARG;
+  e->max_slots = max_slots;
+  return PS_OK;
+}
+
+int ps_set_problem(ps_engine *e, const ps_problem *p) {
+  if (!e || !p) return PS_ERR_ARG;
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  free_problem(e);
+  const int N = p->n_points, F = p->n_fac;
+  if (N < 1 || p->n_lc < 1 || F < 1 || F > kMaxFacets || p->lmax < 0 || p->mmax < 0 || p->mmax > p->lmax ||
+      p->n_poles < 1 || !p->lc_points || !p->lc_inrel || !p->tim || !p->brightness || !p->sig || !p->weight ||
+      !p->ee || !p->ee0 || !p->nor || !p->darea || !p->pleg || !p->fc || !p->fs || !p->dsph || !p->cg_first ||
+      !p->beta_pole || !p->lambda_pole) {
+    e->err = "invalid problem";
+    return PS_ERR_ARG;
+  }
+  int ncoef = 0, ngrp = 0;
+  std::vector<int> grp_m;
+  for (int m = 0; m <= p->mmax; m++)
+    for (int l = m; l <= p->lmax; l++) {
+      ncoef += (m != 0) ? 2 : 1;
+      grp_m.push_back(m);
+      ngrp++;
+    }
+  if (ncoef > kMaxNcoef) {
+    e->err = "too many shape coefficients";
+    return PS_ERR_LIMIT;
+  }
+  long long sum = 0;
+  for (int i = 0; i < p->n_lc; i++) {
+    if (p->lc_points[i] < 1) {
+      e->err = "empty light curve";
+      return PS_ERR_ARG;
+    }
+    sum += p->lc_points[i];
+  }
+  if (sum != N) {
+    e->err = "light-curve point counts do not add up to n_points";
+    return PS_ERR_ARG;
+  }
+  const int ma = ncoef + 8;
+  // free-parameter mask (period_search_BOINC.cpp:789-802, 838-840)
+  std::vector<int> ia(ma, 1), fidx;
+  if (p->onlyrel) ia[0] = 0;
+  ia[ncoef + 2] = p->ia_prd ? 1 : 0;
+  for (int i = 0; i < 3; i++) ia[ncoef + 3 + i] = p->ia_par[i] ? 1 : 0;
+  ia[ncoef + 6] = p->ia_cl ? 1 : 0;
+  ia[ncoef + 7] = 0;
+  for (int l = 0; l < ma; l++)
+    if (ia[l]) fidx.push_back(l);
+  const int mfit = (int)fidx.size();
+  const int ncp = (ncoef + 7) / 8 * 8;
+  ModelFn mk = pick_model(ncp);
+  if (!mk) {
+    e->err = "no model kernel for this coefficient count";
+    return PS_ERR_LIMIT;
+  }
+  // host-side SoA staging
+  std::vector<double> ex(N), ey(N), ez(N), sx(N), sy(N), sz(N);
+  for (int i = 0; i < N; i++) {
+    ex[i] = p->ee[3 * i];
+    ey[i] = p->ee[3 * i + 1];
+    ez[i] = p->ee[3 * i + 2];
+    sx[i] = p->ee0[3 * i];
+    sy[i] = p->ee0[3 * i + 1];
+    sz[i] = p->ee0[3 * i + 2];
+  }
+  std::vector<int> lc_start(p->n_lc + 2), lc_inrel(p->n_lc + 1);
+  lc_start[0] = 0;
+  for (int i = 0; i < p->n_lc; i++) {
+    lc_start[i + 1] = lc_start[i] + p->lc_points[i];
+    lc_inrel[i] = p->lc_inrel[i] ? 1 : 0;
+  }
+  lc_start[p->n_lc + 1] = N + 3;
+  lc_inrel[p->n_lc] = 0;
+  std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
+  for (int f = 0; f < F; f++) {
+    nx[f] = p->nor[3 * f];
+    ny[f] = p->nor[3 * f + 1];
+    nz[f] = p->nor[3 * f + 2];
+    for (int c = 0; c < ncoef; c++) dsph[(size_t)f * ncp + c] = p->dsph[(size_t)f * ncoef + c];
+  }
+  const int L1 = p->lmax + 1, M1 = p->mmax + 1;
+  std::vector<double> cpleg((size_t)ngrp * F), cfc((size_t)M1 * F), cfs((size_t)M1 * F);
+  for (int f = 0; f < F; f++) {
+    int q = 0;
+    for (int m = 0; m <= p->mmax; m++)
+      for (int l = m; l <= p->lmax; l++) cpleg[(size_t)(q++) * F + f] = p->pleg[((size_t)f * L1 + l) * L1 + m];
+    for (int m = 0; m < M1; m++) {
+      cfc[(size_t)m * F + f] = p->fc[(size_t)f * M1 + m];
+      cfs[(size_t)m * F + f] = p->fs[(size_t)f * M1 + m];
+    }
+  }
+  DevProblem &D = e->P;
+  D = DevProblem{};
+  int rc = PS_OK;
+#define UP(src, n, dst) \
+  if ((rc = upload(e, src, n, &D.dst)) != PS_OK) return rc;
+  UP(p->tim, N, tim);
+  UP(p->brightness, N, y);
+  UP(p->sig, N, sig);
+  UP(p->weight, N, w);
+  UP(ex.data(), N, ex);
+  UP(ey.data(), N, ey);
+  UP(ez.data(), N, ez);
+  UP(sx.data(), N, sx);
+  UP(sy.data(), N, sy);
+  UP(sz.data(), N, sz);
+  UP(lc_start.data(), lc_start.size(), lc_start);
+  UP(lc_inrel.data(), lc_inrel.size(), lc_inrel);
+  UP(nx.data(), F, nx);
+  UP(ny.data(), F, ny);
+  UP(nz.data(), F, nz);
+  UP(p->darea, F, darea);
+  UP(dsph.data(), dsph.size(), dsph);
+  UP(cpleg.data(), cpleg.size(), cpleg);
+  UP(cfc.data(), cfc.size(), cfc);
+  UP(cfs.data(), cfs.size(), cfs);
+  UP(grp_m.data(), grp_m.size(), grp_m);
+  UP(fidx.data(), fidx.size(), fidx);
+  UP(p->cg_first, ncoef, cg_first);
+  UP(p->beta_pole, p->n_poles, pole_beta);
+  UP(p->lambda_pole, p->n_poles, pole_lambda);
+#undef UP
+  D.n_points = N;
+  D.n_lc = p->n_lc;
+  D.nfac = F;
+  D.ngrp = ngrp;
+  D.ncoef = ncoef;
+  D.ncp = ncp;
+  D.ma = ma;
+  D.mfit = mfit;
+  D.phi0 = p->phi0;
+  D.alamda_start = p->alamda_start;
+  D.alamda_incr = p->alamda_incr;
+  D.log_cl = log(p->cl);
+  D.par0 = p->par[0];
+  D.par1 = p->par[1];
+  D.par2 = p->par[2];
+  // stop rule (period_search_BOINC.cpp:846-866)
+  D.n_iter_max = 0;
+  D.iter_diff_max = -1;
+  D.n_iter_min = p->n_iter_min;
+  if (p->stop_condition > 1) {
+    D.n_iter_max = (int)p->stop_condition;
+    D.iter_diff_max = 0;
+    D.n_iter_min = 0;
+  }
+  if (p->stop_condition < 1) {
+    D.n_iter_max = 100;
+    D.iter_diff_max = p->stop_condition;
+  }
+  D.ndata = N + 3;
+  D.npad = (N + 3 + 63) / 64 * 64;
+  D.ntiles = (N + kThreads - 1) / kThreads;
+  e->n_poles = p->n_poles;
+  e->model = mk;
+  e->lds_step = step_lds(mfit, F);
+  e->lds_model = model_lds(F);
+  e->lds_neq = neq_lds(mfit, F);
+  const size_t lds_max = 160 * 1024;
+  if (e->lds_step > lds_max || e->lds_neq > lds_max || e->lds_model > lds_max) {
+    e->err = "problem needs more LDS than a CU has";
+    return PS_ERR_LIMIT;
+  }
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(step_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_step));
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(neq_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_neq));
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(mk), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)e->lds_model));
+  e->have_problem = true;
+  return PS_OK;
+}
+
+int ps_dims(const ps_engine *e, int *ncoef, int *ma, int *mfit) {
+  if (!e || !e->have_problem) return PS_ERR_STATE;
+  if (ncoef) *ncoef = e->P.ncoef;
+  if (ma) *ma = e->P.ma;
+  if (mfit) *mfit = e->P.mfit;
+  return PS_OK;
+}
+
+int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits) {
+  if (!e) return PS_ERR_ARG;
+  if (ms) *ms = e->last_ms;
+  if (fits) *fits = e->last_items;
+  return PS_OK;
+}
+
+int ps_fit_batch(ps_engine *e, const double *freq, const int *pole, int n, double conw_r, ps_fit_out *out) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!freq || !pole || !out))) return PS_ERR_ARG;
+  if (n == 0) return PS_OK;
+  for (int i = 0; i < n; i++)
+    if (pole[i] < 0 || pole[i] >= e->n_poles) {
+      e->err = "pole index out of range";
+      return PS_ERR_ARG;
+    }
+  if ((rc = ensure_items(e, n))) return rc;
+  HIP_TRY(hipMemcpyAsync(e->d_freq, freq, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_pole, pole, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+  DevWork W{};
+  W.n_items = n;
+  W.mode = 0;
+  W.freq = e->d_freq;
+  W.pole = e->d_pole;
+  W.conw_r = conw_r;
+  W.out = e->d_out;
+  if ((rc = run_batch(e, W))) return rc;
+  HIP_TRY(hipMemcpyAsync(out, e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return PS_OK;
+}
+
+// Fits of periods n_from..n_to and their pole reduction, rows left on the device.
+static int run_periods(ps_engine *e, double freq_start, double freq_step, int n_from, int n_to, double conw_r,
+                       int precalc) {
+  const int np = n_to - n_from + 1;
+  const size_t items = (size_t)np * e->n_poles;
+  int rc;
+  if ((rc = ensure_items(e, items))) return rc;
+  if ((rc = ensure_rows(e, np))) return rc;
+  hipLaunchKernelGGL(make_items, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, e->stream, freq_start,
+                     freq_step, n_from, np, e->n_poles, e->d_freq, e->d_pole);
+  DevWork W{};
+  W.n_items = (int)items;
+  W.mode = 0;
+  W.freq = e->d_freq;
+  W.pole = e->d_pole;
+  W.conw_r = conw_r;
+  W.out = e->d_out;
+  if ((rc = run_batch(e, W))) return rc;
+  hipLaunchKernelGGL(reduce_poles, dim3((np + 127) / 128), dim3(128), 0, e->stream, e->d_out, np, e->n_poles,
+                     e->P.ndata, precalc, e->d_rows);
+  HIP_TRY(hipGetLastError());
+  return PS_OK;
+}
+
+int ps_scan(ps_engine *e, double freq_start, double freq_step, int n_from, int n_to, double conw_r, ps_row *rows) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (n_from < 1 || n_to < n_from || !rows) return PS_ERR_ARG;
+  if ((rc = run_periods(e, freq_start, freq_step, n_from, n_to, conw_r, 0))) return rc;
+  HIP_TRY(hipMemcpyAsync(rows, e->d_rows, (size_t)(n_to - n_from + 1) * sizeof(ps_row), hipMemcpyDeviceToHost,
+                         e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return PS_OK;
+}
+
+int ps_precalc(ps_engine *e, double freq_start, double freq_end, double freq_step, double *conw_r, int *new_conw) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!conw_r || !new_conw) return PS_ERR_ARG;
+  // period_search_BOINC.cpp:571-712
+  int max_test_periods = 10;
+  int n_iter = (int)((freq_start - freq_end) / freq_step) + 1;
+  if (n_iter < max_test_periods) max_test_periods = n_iter;
+  if (max_test_periods < 1) return PS_ERR_ARG;
+  if ((rc = run_periods(e, freq_start, freq_step, 1, max_test_periods, *conw_r, 1))) return rc;
+  std::vector<ps_row> rows(max_test_periods);
+  HIP_TRY(hipMemcpyAsync(rows.data(), e->d_rows, rows.size() * sizeof(ps_row), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  double sum_dark = 0.0;
+  for (int n = 0; n < max_test_periods; n++) sum_dark = sum_dark + rows[n].dark;
+  double ave_dark = sum_dark / max_test_periods;
+  *new_conw = 0;
+  if (ave_dark < 1.0) *new_conw = 1;
+  if (ave_dark >= 1.0) *conw_r = *conw_r * 2;
+  return PS_OK;
+}
+
+int ps_eval_mrqcof(ps_engine *e, const double *cg, int n, double conw_r, double *alpha, double *beta, double *chisq,
+                   double *chck) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (n <= 0 || !cg || !alpha || !beta || !chisq || !chck) return PS_ERR_ARG;
+  const int ma = e->P.ma, mfit = e->P.mfit;
+  if ((size_t)n > e->ev_cap) {
+    void *bufs[] = {e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
+    for (void *b : bufs)
+      if (b) (void)hipFree(b);
+    e->d_cg = e->d_ea = e->d_eb = e->d_ec = e->d_ek = nullptr;
+    e->ev_cap = 0;
+    HIP_TRY(hipMalloc(&e->d_cg, (size_t)n * kMaxMa * sizeof(double)));
+    HIP_TRY(hipMalloc(&e->d_ea, (size_t)n * kMaxMfit * kMaxMfit * sizeof(double)));
+    HIP_TRY(hipMalloc(&e->d_eb, (size_t)n * kMaxMfit * sizeof(double)));
+    HIP_TRY(hipMalloc(&e->d_ec, (size_t)n * sizeof(double)));
+    HIP_TRY(hipMalloc(&e->d_ek, (size_t)n * 3 * sizeof(double)));
+    e->ev_cap = n;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_cg, cg, (size_t)n * ma * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  DevWork W{};
+  W.n_items = n;
+  W.mode = 1;
+  W.cg_in = e->d_cg;
+  W.conw_r = conw_r;
+  W.ev_alpha = e->d_ea;
+  W.ev_beta = e->d_eb;
+  W.ev_chi = e->d_ec;
+  W.ev_chck = e->d_ek;
+  if ((rc = run_batch(e, W))) return rc;
+  HIP_TRY(hipMemcpyAsync(alpha, e->d_ea, (size_t)n * mfit * mfit * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(beta, e->d_eb, (size_t)n * mfit * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(chisq, e->d_ec, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(chck, e->d_ek, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return PS_OK;
+}
+
+}  // extern "C"
