@@ -158,6 +158,12 @@ int ps_dims(const ps_engine *e, int *ncoef, int *ma, int *mfit);
  * engine's stream, and the number of fits it ran. */
 int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits);
 
+/* Per-launch timing of the model kernel (the dominant kernel) with HIP
+ * events on the engine's stream, and a count of model evaluations.
+ * ps_set_profiling resets the counters. */
+int ps_set_profiling(ps_engine *e, int on);
+int ps_kernel_stats(const ps_engine *e, double *model_ms, long long *model_launches, long long *evals);
+
 /* ---- host-side driver helpers ---------------------------------------- */
 /* Parsed and normalised period_search_in (period_search_BOINC.cpp:277-485). */
 typedef struct ps_input ps_input;

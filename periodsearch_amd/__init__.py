@@ -80,6 +80,8 @@ def _load() -> C.CDLL:
         "ps_eval_mrqcof": (C.c_int, [vp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
         "ps_dims": (C.c_int, [vp, _ip, _ip, _ip]),
         "ps_last_kernel_ms": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
+        "ps_set_profiling": (C.c_int, [vp, C.c_int]),
+        "ps_kernel_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ps_input_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
         "ps_input_free": (None, [vp]),
         "ps_input_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
@@ -99,7 +101,7 @@ lib = _load()
 
 # Symbols declared in include/periodsearch.h (checked by the CPU test suite).
 EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_set_slots", "ps_set_problem",
-            "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
+            "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms", "ps_set_profiling", "ps_kernel_stats",
             "ps_input_read", "ps_input_free", "ps_input_problem", "ps_input_scalars", "ps_format_row",
             "ps_shape_basis")
 
@@ -239,6 +241,16 @@ class Engine:
         self._check(lib.ps_eval_mrqcof(self._h, _d(cg), n, conw_r, _d(alpha), _d(beta), _d(chi), _d(chck)),
                     "ps_eval_mrqcof")
         return alpha, beta, chi, chck
+
+    def set_profiling(self, on: bool = True):
+        self._check(lib.ps_set_profiling(self._h, 1 if on else 0), "ps_set_profiling")
+
+    def kernel_stats(self):
+        ms = C.c_double()
+        n = C.c_longlong()
+        ev = C.c_longlong()
+        lib.ps_kernel_stats(self._h, C.byref(ms), C.byref(n), C.byref(ev))
+        return dict(model_ms=ms.value, model_launches=n.value, evals=ev.value)
 
     def last_kernel_ms(self):
         ms = C.c_double()

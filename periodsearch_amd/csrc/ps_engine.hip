@@ -124,7 +124,7 @@ struct DevWork {
   double conw_r;
   ps_fit_out *out;          // [n_items] (mode 0)
   double *ev_alpha, *ev_beta, *ev_chi, *ev_chck;  // (mode 1)
-  int *queue, *done;
+  int *queue, *done;   // done[1] counts model evaluations
   SlotState *st;
   double *mem;
   long long slot_doubles;
@@ -766,6 +766,7 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) 
   const int sel = S.sel;
   double *atr = mem + (sel ? M.alpha0 : M.alpha1), *btr = mem + (sel ? M.beta0 : M.beta1);
   if (eval) {
+    if (tid == 0) atomicAdd(W.done + 1, 1);
     for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
     __syncthreads();
     eval_curv(P, x, area, g);
@@ -872,7 +873,8 @@ __global__ void reset_batch(SlotState *st, int nslots, int *queue, int *done) {
   }
   if (i == 0) {
     *queue = 0;
-    *done = 0;
+    done[0] = 0;
+    done[1] = 0;
   }
 }
 
@@ -980,6 +982,12 @@ struct ps_engine {
   size_t ev_cap = 0;
   double last_ms = 0;
   long long last_items = 0, last_iters = 0;
+  // optional per-launch timing of the model kernel (HIP events on the
+  // engine's stream)
+  bool profile = false;
+  std::vector<hipEvent_t> pev;
+  double model_ms = 0;
+  long long model_launches = 0, evals = 0;
 };
 
 namespace {
@@ -1063,18 +1071,32 @@ int run_batch(ps_engine *e, DevWork W) {
   const long long rounds = (W.n_items + nslots - 1) / nslots + 1;
   const long long guard = rounds * per_fit * 2 + 64;
   const int poll = 8;
+  if (e->profile && e->pev.empty()) {
+    e->pev.resize(2 * poll);
+    for (auto &ev : e->pev) HIP_TRY(hipEventCreate(&ev));
+  }
   HIP_TRY(hipEventRecord(e->ev0, e->stream));
   long long it = 0;
   while (true) {
     for (int k = 0; k < poll; k++) {
       hipLaunchKernelGGL(step_kernel, dim3(nslots), dim3(kThreads), e->lds_step, e->stream, P, W);
+      if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k], e->stream));
       hipLaunchKernelGGL(e->model, dim3(nslots * P.ntiles), dim3(kThreads), e->lds_model, e->stream, P, W);
+      if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k + 1], e->stream));
       hipLaunchKernelGGL(neq_kernel, dim3(nslots), dim3(kThreads), e->lds_neq, e->stream, P, W);
     }
     HIP_TRY(hipGetLastError());
     it += poll;
-    HIP_TRY(hipMemcpyAsync(e->host_done, W.done, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->host_done, W.done, 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->profile) {
+      for (int k = 0; k < poll; k++) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * k], e->pev[2 * k + 1]));
+        e->model_ms += ms;
+      }
+      e->model_launches += poll;
+    }
     if (*e->host_done >= W.n_items) break;
     if (it > guard) {
       e->err = "fit batch did not finish within its iteration bound";
@@ -1088,6 +1110,7 @@ int run_batch(ps_engine *e, DevWork W) {
   e->last_ms = ms;
   e->last_items = W.n_items;
   e->last_iters = it;
+  e->evals += e->host_done[1];
   return PS_OK;
 }
 
@@ -1115,7 +1138,7 @@ int ps_create(int device, ps_engine **out) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
       hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     delete e;
     return PS_ERR_HIP;
   }
@@ -1139,6 +1162,7 @@ int ps_destroy(ps_engine *e) {
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (e->host_done) (void)hipHostFree(e->host_done);
+  for (auto ev : e->pev) (void)hipEventDestroy(ev);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1348,6 +1372,23 @@ int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits) {
   if (!e) return PS_ERR_ARG;
   if (ms) *ms = e->last_ms;
   if (fits) *fits = e->last_items;
+  return PS_OK;
+}
+
+int ps_set_profiling(ps_engine *e, int on) {
+  if (!e) return PS_ERR_ARG;
+  e->profile = on != 0;
+  e->model_ms = 0;
+  e->model_launches = 0;
+  e->evals = 0;
+  return PS_OK;
+}
+
+int ps_kernel_stats(const ps_engine *e, double *model_ms, long long *model_launches, long long *evals) {
+  if (!e) return PS_ERR_ARG;
+  if (model_ms) *model_ms = e->model_ms;
+  if (model_launches) *model_launches = e->model_launches;
+  if (evals) *evals = e->evals;
   return PS_OK;
 }
 
