@@ -68,6 +68,7 @@ struct DevProblem {
   // facets
   const double *nx, *ny, *nz, *darea;
   const double *dsph;     // [F][ncp], zero-padded columns
+  const float *nxf, *nyf, *nzf, *dsphf;  // FP32 copies for the reduced-precision model
   const double *cpleg;    // [ngrp][F]: P_l^m of coefficient group q at facet f
   const double *cfc;      // [mmax+1][F]
   const double *cfs;      // [mmax+1][F]
@@ -263,21 +264,24 @@ __device__ void rotation_derivs(const DevProblem &P, int p, const double *blm, c
 // NCB = accumulator count (ncoef rounded up to 8; Dsph is zero-padded).
 template <int NCB>
 __device__ void model_tile(const DevProblem &P, int p0, const double *a, const double *blm, const double *area,
-                           double *J) {
+                           double *J, PointGeom *gst) {
   const int lane_p = p0 + (int)threadIdx.x;
   const int wave_first = p0 + ((int)threadIdx.x & ~63);
   if (wave_first >= P.n_points) return;  // whole wave idle
   const int p = lane_p < P.n_points ? lane_p : P.n_points - 1;
-  PointGeom G;
-  point_geometry(P, p, a, blm, G);
+  {
+    PointGeom G;
+    point_geometry(P, p, a, blm, G);
+    gst[threadIdx.x] = G;  // parked in LDS while the facet loop runs
+  }
   const double cl = exp(a[P.ma - 2]);
   const double cls = a[P.ma - 1];
   double S[NCB];
 #pragma unroll
   for (int c = 0; c < NCB; c++) S[c] = 0;
   double br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
-  const double e_0 = G.e[0], e_1 = G.e[1], e_2 = G.e[2];
-  const double s_0 = G.e0[0], s_1 = G.e0[1], s_2 = G.e0[2];
+  const double e_0 = gst[threadIdx.x].e[0], e_1 = gst[threadIdx.x].e[1], e_2 = gst[threadIdx.x].e[2];
+  const double s_0 = gst[threadIdx.x].e0[0], s_1 = gst[threadIdx.x].e0[1], s_2 = gst[threadIdx.x].e0[2];
   const double *__restrict__ nxp = P.nx;
   const double *__restrict__ nyp = P.ny;
   const double *__restrict__ nzp = P.nz;
@@ -312,6 +316,7 @@ __device__ void model_tile(const DevProblem &P, int p0, const double *a, const d
     }
   }
   if (lane_p >= P.n_points) return;
+  const PointGeom G = gst[threadIdx.x];
   const int ld = P.npad;
   const double sc = G.scale;
 #pragma unroll
@@ -325,6 +330,84 @@ __device__ void model_tile(const DevProblem &P, int p0, const double *a, const d
   J[(size_t)(P.ncoef + 6) * ld + p] = sc * scl * cl;
   J[(size_t)(P.ncoef + 7) * ld + p] = sc * scls;
   J[(size_t)P.ma * ld + p] = br * sc;
+}
+
+// Reduced-precision pass 1 (PS_PREC_FP32): the phase reduction, rotation
+// matrices and phase function stay FP64 (point_geometry); the facet loop and
+// the brightness-derivative contraction run in FP32; rows are stored as FP64
+// so the normal equations, solve, chi2 and the LM decisions stay FP64.
+template <int NCB>
+__device__ void model_tile_f32(const DevProblem &P, int p0, const double *a, const double *blm, const double *area,
+                               double *J, PointGeom *gst) {
+  const int lane_p = p0 + (int)threadIdx.x;
+  const int wave_first = p0 + ((int)threadIdx.x & ~63);
+  if (wave_first >= P.n_points) return;
+  const int p = lane_p < P.n_points ? lane_p : P.n_points - 1;
+  {
+    PointGeom G;
+    point_geometry(P, p, a, blm, G);
+    gst[threadIdx.x] = G;  // parked in LDS while the facet loop runs
+  }
+  const float cl = (float)exp(a[P.ma - 2]);
+  const float cls = (float)a[P.ma - 1];
+  const float tiny = (float)kTiny;
+  float S[NCB];
+#pragma unroll
+  for (int c = 0; c < NCB; c++) S[c] = 0.f;
+  float br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
+  const float e_0 = (float)gst[threadIdx.x].e[0], e_1 = (float)gst[threadIdx.x].e[1],
+              e_2 = (float)gst[threadIdx.x].e[2];
+  const float s_0 = (float)gst[threadIdx.x].e0[0], s_1 = (float)gst[threadIdx.x].e0[1],
+              s_2 = (float)gst[threadIdx.x].e0[2];
+  const float *__restrict__ nxp = P.nxf;
+  const float *__restrict__ nyp = P.nyf;
+  const float *__restrict__ nzp = P.nzf;
+  const float *__restrict__ dsp = P.dsphf;
+#pragma unroll 1
+  for (int f = 0; f < P.nfac; f++) {
+    const float n0 = nxp[f], n1 = nyp[f], n2 = nzp[f];
+    const float mu = e_0 * n0 + e_1 * n1 + e_2 * n2;
+    const float mu0 = s_0 * n0 + s_1 * n1 + s_2 * n2;
+    if (mu > tiny && mu0 > tiny) {
+      const float af = (float)area[f];
+      const float rd = 1.0f / (mu + mu0);
+      const float mm = mu * mu0;
+      const float s = mm * (cl + cls * rd);
+      const float wgt = af * s;
+      br += wgt;
+      const float q0 = mu0 * rd, q1 = mu * rd;
+      const float ad = af * (cls * q0 * q0 + cl * mu0);
+      const float ad0 = af * (cls * q1 * q1 + cl * mu);
+      v0 += ad * n0;
+      v1 += ad * n1;
+      v2 += ad * n2;
+      u0 += ad0 * n0;
+      u1 += ad0 * n1;
+      u2 += ad0 * n2;
+      const float am = af * mm;
+      scl += am;
+      scls += am * rd;
+      const float *drow = dsp + (size_t)f * NCB;
+#pragma unroll
+      for (int c = 0; c < NCB; c++) S[c] += wgt * drow[c];
+    }
+  }
+  if (lane_p >= P.n_points) return;
+  const PointGeom G = gst[threadIdx.x];
+  const int ld = P.npad;
+  const double sc = G.scale;
+#pragma unroll
+  for (int c = 0; c < NCB; c++)
+    if (c < P.ncoef) J[(size_t)c * ld + p] = sc * (double)S[c];
+  const double V[3] = {v0, v1, v2}, U[3] = {u0, u1, u2};
+  double r[3];
+  rotation_derivs(P, p, blm, G, V, U, r);
+  for (int k = 0; k < 3; k++) J[(size_t)(P.ncoef + k) * ld + p] = sc * r[k];
+  const double brd = br;
+  for (int i = 0; i < 3; i++) J[(size_t)(P.ncoef + 3 + i) * ld + p] = brd * G.dphp[i];
+  J[(size_t)(P.ncoef + 6) * ld + p] = sc * (double)scl * (double)cl;
+  J[(size_t)(P.ncoef + 7) * ld + p] = sc * (double)scls;
+  J[(size_t)P.ma * ld + p] = brd * sc;
 }
 
 // The three convexity rows (period_search/conv.c:12-29) at points
@@ -731,7 +814,7 @@ __global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W)
 }
 
 // model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
-template <int NCB>
+template <int NCB, bool F32>
 __global__ void __launch_bounds__(kThreads) model_kernel(DevProblem P, DevWork W) {
   extern __shared__ double sm[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
@@ -739,12 +822,16 @@ __global__ void __launch_bounds__(kThreads) model_kernel(DevProblem P, DevWork W
   const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
   double *x = sm, *blm = x + kMaxMa, *area = blm + 32, *g = area + P.nfac;
+  PointGeom *gst = reinterpret_cast<PointGeom *>(g + P.nfac);
   for (int l = threadIdx.x; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
   __syncthreads();
   eval_curv(P, x, area, g);
   if (threadIdx.x == 0) eval_blmatrix(x[P.ncoef], x[P.ncoef + 1], blm);
   __syncthreads();
-  model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J);
+  if constexpr (F32)
+    model_tile_f32<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
+  else
+    model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
 }
 
 // neq_kernel: one workgroup per slot.  Convexity rows, normal equations at
@@ -921,24 +1008,25 @@ __global__ void make_items(double freq_start, double freq_step, int n_from, int 
 
 using ModelFn = void (*)(DevProblem, DevWork);
 
+template <bool F32>
 ModelFn pick_model(int ncp) {
   switch (ncp) {
-    case 8: return model_kernel<8>;
-    case 16: return model_kernel<16>;
-    case 24: return model_kernel<24>;
-    case 32: return model_kernel<32>;
-    case 40: return model_kernel<40>;
-    case 48: return model_kernel<48>;
-    case 56: return model_kernel<56>;
-    case 64: return model_kernel<64>;
-    case 72: return model_kernel<72>;
-    case 80: return model_kernel<80>;
-    case 88: return model_kernel<88>;
-    case 96: return model_kernel<96>;
-    case 104: return model_kernel<104>;
-    case 112: return model_kernel<112>;
-    case 120: return model_kernel<120>;
-    case 128: return model_kernel<128>;
+    case 8: return model_kernel<8, F32>;
+    case 16: return model_kernel<16, F32>;
+    case 24: return model_kernel<24, F32>;
+    case 32: return model_kernel<32, F32>;
+    case 40: return model_kernel<40, F32>;
+    case 48: return model_kernel<48, F32>;
+    case 56: return model_kernel<56, F32>;
+    case 64: return model_kernel<64, F32>;
+    case 72: return model_kernel<72, F32>;
+    case 80: return model_kernel<80, F32>;
+    case 88: return model_kernel<88, F32>;
+    case 96: return model_kernel<96, F32>;
+    case 104: return model_kernel<104, F32>;
+    case 112: return model_kernel<112, F32>;
+    case 120: return model_kernel<120, F32>;
+    case 128: return model_kernel<128, F32>;
     default: return nullptr;
   }
 }
@@ -946,7 +1034,9 @@ ModelFn pick_model(int ncp) {
 size_t step_lds(int mfit, int nfac) {
   return ((size_t)mfit * mfit + 3 * kMaxMfit + 2 * kWaves + 2 * (size_t)nfac + kMaxMfit / 2 + 2) * sizeof(double);
 }
-size_t model_lds(int nfac) { return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double); }
+size_t model_lds(int nfac) {
+  return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
+}
 size_t neq_lds(int mfit, int nfac) {
   return ((size_t)2 * kChunk * mfit + 3 * kChunk + 2 * (size_t)nfac + 2 * kMaxMa + 1 + 8) * sizeof(double);
 }
@@ -1174,8 +1264,8 @@ const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "nul
 
 int ps_set_precision(ps_engine *e, int precision) {
   if (!e) return PS_ERR_ARG;
-  if (precision != PS_PREC_FP64) {
-    e->err = "only PS_PREC_FP64 is available in this build";
+  if (precision != PS_PREC_FP64 && precision != PS_PREC_FP32) {
+    e->err = "unknown precision";
     return PS_ERR_ARG;
   }
   e->precision = precision;
@@ -1237,7 +1327,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
     if (ia[l]) fidx.push_back(l);
   const int mfit = (int)fidx.size();
   const int ncp = (ncoef + 7) / 8 * 8;
-  ModelFn mk = pick_model(ncp);
+  ModelFn mk = e->precision == PS_PREC_FP32 ? pick_model<true>(ncp) : pick_model<false>(ncp);
   if (!mk) {
     e->err = "no model kernel for this coefficient count";
     return PS_ERR_LIMIT;
@@ -1261,11 +1351,16 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   lc_start[p->n_lc + 1] = N + 3;
   lc_inrel[p->n_lc] = 0;
   std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
+  std::vector<float> nxf(F), nyf(F), nzf(F), dsphf((size_t)F * ncp, 0.0f);
   for (int f = 0; f < F; f++) {
     nx[f] = p->nor[3 * f];
     ny[f] = p->nor[3 * f + 1];
     nz[f] = p->nor[3 * f + 2];
     for (int c = 0; c < ncoef; c++) dsph[(size_t)f * ncp + c] = p->dsph[(size_t)f * ncoef + c];
+    nxf[f] = (float)nx[f];
+    nyf[f] = (float)ny[f];
+    nzf[f] = (float)nz[f];
+    for (int c = 0; c < ncp; c++) dsphf[(size_t)f * ncp + c] = (float)dsph[(size_t)f * ncp + c];
   }
   const int L1 = p->lmax + 1, M1 = p->mmax + 1;
   std::vector<double> cpleg((size_t)ngrp * F), cfc((size_t)M1 * F), cfs((size_t)M1 * F);
@@ -1300,6 +1395,10 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   UP(nz.data(), F, nz);
   UP(p->darea, F, darea);
   UP(dsph.data(), dsph.size(), dsph);
+  UP(nxf.data(), F, nxf);
+  UP(nyf.data(), F, nyf);
+  UP(nzf.data(), F, nzf);
+  UP(dsphf.data(), dsphf.size(), dsphf);
   UP(cpleg.data(), cpleg.size(), cpleg);
   UP(cfc.data(), cfc.size(), cfc);
   UP(cfs.data(), cfs.size(), cfs);

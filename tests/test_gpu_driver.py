@@ -1,0 +1,58 @@
+"""The command-line driver (period_search_amd) end to end on the GPU: output
+rows against the oracle's driver, the validator rule, and checkpoint/resume
+(period_search_BOINC.cpp:85-101, 237-251)."""
+import os
+import subprocess
+
+import pytest
+
+import oracle
+from periodsearch_amd import DRIVER_PATH, synth
+from validator_rule import compare, read_rows
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def short_scan(tmp_path_factory):
+    d = tmp_path_factory.mktemp("drv")
+    path = str(d / "period_search_in")
+    # 6 trial periods: 1/P_end = 1/7 - 5 * freq_step (8-yr span, step coef 0.5)
+    ast = synth.make_asteroid(0, **{k: v for k, v in synth.WORKLOADS["small"].items()
+                                    if k not in ("seed", "per_start", "per_end")})
+    synth.write_input(path, ast, 7.0, 7.0009)
+    return str(d), path
+
+
+def _run_driver(wd, extra=()):
+    return subprocess.run([DRIVER_PATH, "--batch", "2", *extra], cwd=wd, capture_output=True, text=True,
+                          timeout=300)
+
+
+def test_driver_matches_oracle_and_resumes(short_scan):
+    wd, path = short_scan
+    r = _run_driver(wd)
+    assert r.returncode == 0, r.stderr
+    gpu = open(os.path.join(wd, "period_search_out")).read()
+    ref_out = os.path.join(wd, "oracle_out")
+    oracle.run_oracle_main(path, ref_out, timeout=600)
+    orc = open(ref_out).read()
+    g_rows, o_rows = read_rows(gpu), read_rows(orc)
+    assert len(g_rows) == len(o_rows) >= 5
+    assert compare(gpu, orc)
+    for g, o in zip(g_rows, o_rows):
+        assert abs(g[0] - o[0]) <= 1e-7 * o[0]       # period
+        assert abs(g[1] - o[1]) <= 2e-6               # rms printed with 6 decimals
+        assert g[3] == o[3]                           # dark facet / conw column
+        assert abs(g[4] - o[4]) <= 1 and abs(g[5] - o[5]) <= 1
+    state = open(os.path.join(wd, "period_search_state")).read().split()
+    assert int(state[0]) == len(g_rows) and int(state[1]) == 1
+    # resume: a base-format checkpoint after 3 rows and a truncated output
+    lines = gpu.splitlines(keepends=True)
+    with open(os.path.join(wd, "period_search_out"), "w") as f:
+        f.writelines(lines[:3])
+    with open(os.path.join(wd, "period_search_state"), "w") as f:
+        f.write("3")
+    r = _run_driver(wd)
+    assert r.returncode == 0, r.stderr
+    assert open(os.path.join(wd, "period_search_out")).read() == gpu

@@ -137,3 +137,30 @@ def test_reference_sample_rows(golden_dir):
         assert abs(got[0] - exp[0]) < 1e-7
         assert abs(got[1] - exp[1]) <= 2e-6 and abs(got[2] - exp[2]) <= 2e-5
         assert got[4:] == exp[4:]
+
+
+def test_fp32_model_fits_within_tolerance(small_input):
+    """Reduced-precision form: facet stage and contraction in FP32 (phase
+    reduction, normal equations, solve, chi2 in FP64).  Final chi2 within 1e-4
+    relative of the FP64 oracle for at least 90% of fits; the fitted period
+    within 1e-6 relative."""
+    import oracle
+    from periodsearch_amd import PS_PREC_FP32, Engine, Input
+
+    inp = Input(small_input)
+    eng = Engine(0)
+    eng.set_precision(PS_PREC_FP32)
+    eng.set_problem(inp.problem, keep=inp)
+    orc = oracle.Oracle(small_input)
+    orc.prepare(inp.conw_r0)
+    f = inp.freq_start - 4 * inp.freq_step
+    out = eng.fit_batch([f] * 10, list(range(10)), inp.conw_r0)
+    rel = []
+    for m in range(10):
+        r = orc.fit(f, m + 1)
+        rel.append(_rel(out["chisq"][m], r.chisq))
+        assert _rel(out["prd"][m], r.prd) < 1e-6, m
+    rel = np.array(rel)
+    print("FP32 model: chi2 relative error median %.2e max %.2e" % (np.median(rel), rel.max()))
+    assert (rel < 1e-4).mean() >= 0.9
+    eng.close()
