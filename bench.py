@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector = matrix), dense
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector = matrix), dense
 WORKLOAD = "wu"
 
 
@@ -67,19 +68,34 @@ def main() -> int:
     ap.add_argument("--periods", type=int, default=256, help="trial periods per step")
     ap.add_argument("--slots", type=int, default=0, help="resident fits (0 = engine default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--prec", choices=("fp64", "fp32"), default="fp64")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_mib": args.calibrate_copy, "copies": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the period index shards across GPUs by contiguous ranges, see DESIGN.md)", file=sys.stderr)
         return 2
 
-    from periodsearch_amd import Engine, Input, synth
+    from periodsearch_amd import PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
 
     tmp = tempfile.mkdtemp(prefix="psbench_")
     path = os.path.join(tmp, "period_search_in")
     synth.write_workload(WORKLOAD, path)
     inp = Input(path)
     eng = Engine(0)
+    eng.set_precision(PS_PREC_FP32 if args.prec == "fp32" else PS_PREC_FP64)
     if args.slots:
         eng.set_slots(args.slots)
     eng.set_problem(inp.problem, keep=inp)
@@ -109,6 +125,7 @@ def main() -> int:
     # model evaluation (masked facets counted, as the reference's dense loop)
     flop_eval = 2.0 * N * F * nc
     achieved = (st["evals"] * flop_eval) / (st["model_ms"] * 1e-3) / 1e12 if st["model_ms"] > 0 else 0.0
+    peak = FP64_PEAK_TFLOPS if args.prec == "fp64" else FP32_PEAK_TFLOPS
     iters = 50
     flop_fit = (iters + 1) * (2.0 * N * F * nc + (N + 3) * mfit * (mfit + 1))
     out = {
@@ -122,13 +139,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.prec == "fp64" else "f32 model / f64 solve",
         "data": "synthetic",
         "config": {"workload": "1080 points (6x80 dense + 600 sparse, 15 yr, 5% noise), lmax=mmax=6, "
                                "nrows=8 (512 facets), 2-100 h, stop 50, 10 poles",
-                   "periods_per_step": P, "fits_per_step": P * 10, "precision": "fp64", "form": "plain FP64"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                   "periods_per_step": P, "fits_per_step": P * 10, "precision": args.prec,
+                   "form": "plain FP64" if args.prec == "fp64" else "FP32 model stage, FP64 solve and chi2"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": "model_kernel", "model_ms_total": round(st["model_ms"], 3),
                      "model_launches": st["model_launches"], "evals": st["evals"],
                      "flop_per_eval": flop_eval},

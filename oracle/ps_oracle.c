@@ -944,6 +944,10 @@ void orc_prepare_round(orc_ctx *c, double conw_r) {
   if (c->onlyrel == 1) c->ia[1] = 0;
   c->ia[c->Ncoef + 3 + c->Nphpar + 1] = c->ia_cl;
   c->ia[c->Ncoef + 3 + c->Nphpar + 2] = 0;
+  /* The reference sets the phase-parameter flags inside every fit
+   * (period_search_BOINC.cpp:838-840, always to ia_par); setting them here
+   * as well lets a single mrqcof() be evaluated before any fit has run. */
+  for (i = 1; i <= c->Nphpar; i++) c->ia[c->Ncoef + 3 + i] = c->ia_par[i];
 }
 
 void orc_end_round(orc_ctx *c) { c->ndata = c->ndata_real; }

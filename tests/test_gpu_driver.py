@@ -17,9 +17,11 @@ pytestmark = pytest.mark.gpu
 def short_scan(tmp_path_factory):
     d = tmp_path_factory.mktemp("drv")
     path = str(d / "period_search_in")
-    # 6 trial periods: 1/P_end = 1/7 - 5 * freq_step (8-yr span, step coef 0.5)
-    ast = synth.make_asteroid(0, **{k: v for k, v in synth.WORKLOADS["small"].items()
-                                    if k not in ("seed", "per_start", "per_end")})
+    # 6 trial periods: 1/P_end = 1/7 - 5 * freq_step (8-yr span, step coef 0.5);
+    # lmax=mmax=3 keeps the oracle's CPU run short
+    cfg = {k: v for k, v in synth.WORKLOADS["small"].items() if k not in ("seed", "per_start", "per_end")}
+    cfg.update(lmax=3, mmax=3)
+    ast = synth.make_asteroid(0, **cfg)
     synth.write_input(path, ast, 7.0, 7.0009)
     return str(d), path
 
