@@ -23,7 +23,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpsamd.so")
+# PSAMD_LIB selects another build of the same library (kernel variants).
+LIB_PATH = os.environ.get("PSAMD_LIB") or os.path.join(HERE, "libpsamd.so")
 DRIVER_PATH = os.path.join(HERE, "period_search_amd")
 
 PS_PREC_FP64 = 0

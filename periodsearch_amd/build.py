@@ -37,24 +37,30 @@ def _run(cmd: list[str]) -> None:
     print(f"[build] {os.path.basename(cmd[-1]) if cmd else ''} ok in {dt:.1f}s", flush=True)
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, defines=(), lib_out: str | None = None, driver: bool = True) -> str:
+    """Compile the library (and the driver).  ``defines`` are extra -D flags
+    for kernel variants; ``lib_out`` writes the library elsewhere."""
     objs = []
     os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
-    eng_o = os.path.join(HERE, "_obj", "ps_engine.o")
+    tag = "_".join(d.replace("=", "") for d in defines)
+    eng_o = os.path.join(HERE, "_obj", f"ps_engine{tag}.o")
     host_o = os.path.join(HERE, "_obj", "ps_host.o")
     _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-I", INCLUDE, *DEVICE_FLAGS,
-          "-c", "-o", eng_o, os.path.join(CSRC, "ps_engine.hip")])
+          *[f"-D{d}" for d in defines], "-c", "-o", eng_o, os.path.join(CSRC, "ps_engine.hip")])
     # no sin/cos fusion by the compiler (see csrc/ps_host.cpp)
     _run(["g++", "-O3", "-fPIC", "-std=c++17", "-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sincos",
           "-I", INCLUDE, "-c", "-o", host_o,
           os.path.join(CSRC, "ps_host.cpp")])
     objs = [eng_o, host_o]
-    tmp = LIB + ".tmp"
+    lib = lib_out or LIB
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+    if not driver:
+        return lib
     _run(["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-o", EXE, os.path.join(CSRC, "ps_main.cpp"),
           "-L", HERE, "-lpsamd", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
-    return LIB
+    return lib
 
 
 if __name__ == "__main__":

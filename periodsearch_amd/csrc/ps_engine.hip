@@ -31,6 +31,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -45,6 +46,8 @@ constexpr int kMaxNcoef = 128;
 constexpr int kMaxMa = kMaxNcoef + 8;
 constexpr int kMaxMfit = kMaxMa;
 constexpr int kChunk = 32;         // points staged per normal-equation chunk
+constexpr int kTilePts = 64;       // points per MFMA model tile (4 row blocks of 16)
+constexpr int kFacetChunk = 32;    // facets per MFMA model chunk (8 k-steps of 4)
 constexpr int kMaxFacets = 1000;
 constexpr double kTiny = 1e-8;     // period_search/constants.h:10
 constexpr double kPi = 3.14159265358979323846;
@@ -69,6 +72,7 @@ struct DevProblem {
   const double *nx, *ny, *nz, *darea;
   const double *dsph;     // [F][ncp], zero-padded columns
   const float *nxf, *nyf, *nzf, *dsphf;  // FP32 copies for the reduced-precision model
+  const double4 *nor4;    // [F] (nx, ny, nz, 0): one 32-byte scalar load per facet
   const double *cpleg;    // [ngrp][F]: P_l^m of coefficient group q at facet f
   const double *cfc;      // [mmax+1][F]
   const double *cfs;      // [mmax+1][F]
@@ -82,7 +86,8 @@ struct DevProblem {
   double iter_diff_max;
   int ndata;   // n_points + 3
   int npad;    // Jacobian leading dimension (>= n_points + 3)
-  int ntiles;  // point tiles of kThreads
+  int ntiles;    // point tiles of kThreads (VALU model kernel)
+  int ntiles64;  // point tiles of kTilePts (MFMA model kernel)
 };
 
 // Per-slot LM state (scalars).  Vectors and matrices live in the slot's
@@ -97,12 +102,14 @@ enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_DRAINED = 3 };
 
 // Offsets (in doubles) inside one slot's memory block.
 struct SlotMem {
-  long long J, alpha0, alpha1, beta0, beta1, a, atry, alast, total;
+  long long J, alpha0, alpha1, beta0, beta1, a, atry, alast, area, g, total;
 };
 
 __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
   SlotMem m;
   long long o = 0;
+  m.area = o; o += kMaxFacets;   // curv() at atry: Area, g (written by step_kernel)
+  m.g = o; o += kMaxFacets;
   m.J = o; o += (long long)(ma + 1) * npad;
   m.alpha0 = o; o += (long long)mfit * mfit;
   m.alpha1 = o; o += (long long)mfit * mfit;
@@ -167,6 +174,22 @@ __device__ void eval_blmatrix(double bet, double lam, double *blm) {
   D2[0] = -cb * sl; D2[1] = cb * cl;  D2[2] = 0;
   D2[3] = -cl;      D2[4] = -sl;      D2[5] = 0;
   D2[6] = -sb * sl; D2[7] = sb * cl;  D2[8] = 0;
+}
+
+// 1/x for x in (2e-8, 2]: hardware reciprocal estimate refined by two
+// Newton-Raphson steps r <- r + r(1 - x r) (quadratic convergence from the
+// ~2^-23 estimate to below 1 ulp); the exact IEEE division is not needed in
+// the facet loop, whose sums are regrouped anyway.
+__device__ inline double recip(double x) {
+#ifdef PS_EXACT_DIV
+  return 1.0 / x;
+#else
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#endif
 }
 
 // Per-point geometry, part 1: the phase function (period_search/phasec.c:11-24)
@@ -282,18 +305,17 @@ __device__ void model_tile(const DevProblem &P, int p0, const double *a, const d
   double br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
   const double e_0 = gst[threadIdx.x].e[0], e_1 = gst[threadIdx.x].e[1], e_2 = gst[threadIdx.x].e[2];
   const double s_0 = gst[threadIdx.x].e0[0], s_1 = gst[threadIdx.x].e0[1], s_2 = gst[threadIdx.x].e0[2];
-  const double *__restrict__ nxp = P.nx;
-  const double *__restrict__ nyp = P.ny;
-  const double *__restrict__ nzp = P.nz;
+  const double4 *__restrict__ n4 = P.nor4;
   const double *__restrict__ dsp = P.dsph;
 #pragma unroll 1
   for (int f = 0; f < P.nfac; f++) {
-    const double n0 = nxp[f], n1 = nyp[f], n2 = nzp[f];
+    const double4 nv = n4[f];
+    const double n0 = nv.x, n1 = nv.y, n2 = nv.z;
     const double mu = e_0 * n0 + e_1 * n1 + e_2 * n2;
     const double mu0 = s_0 * n0 + s_1 * n1 + s_2 * n2;
     if (mu > kTiny && mu0 > kTiny) {
       const double af = area[f];
-      const double rd = 1.0 / (mu + mu0);
+      const double rd = recip(mu + mu0);
       const double mm = mu * mu0;
       const double s = mm * (cl + cls * rd);
       const double wgt = af * s;
@@ -774,6 +796,8 @@ __global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W)
       init_params(P, W.freq[item], W.pole[item], a);
     }
     for (int l = tid; l < ma; l += kThreads) atry[l] = a[l];
+    __syncthreads();
+    eval_curv(P, atry, mem + M.area, mem + M.g);
     if (tid == 0) {
       S.item = item;
       S.phase = PH_INIT;
@@ -806,6 +830,8 @@ __global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W)
       const int l = P.fidx[jf];
       atry[l] = a[l] + da[jf];
     }
+    __syncthreads();
+    eval_curv(P, atry, mem + M.area, mem + M.g);
   }
   if (tid == 0) {
     S.eval = err == 0 ? 1 : 0;
@@ -814,8 +840,11 @@ __global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W)
 }
 
 // model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
+#ifndef PS_MODEL_MIN_WAVES
+#define PS_MODEL_MIN_WAVES 1
+#endif
 template <int NCB, bool F32>
-__global__ void __launch_bounds__(kThreads) model_kernel(DevProblem P, DevWork W) {
+__global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(DevProblem P, DevWork W) {
   extern __shared__ double sm[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
@@ -824,14 +853,168 @@ __global__ void __launch_bounds__(kThreads) model_kernel(DevProblem P, DevWork W
   double *x = sm, *blm = x + kMaxMa, *area = blm + 32, *g = area + P.nfac;
   PointGeom *gst = reinterpret_cast<PointGeom *>(g + P.nfac);
   for (int l = threadIdx.x; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
-  __syncthreads();
-  eval_curv(P, x, area, g);
+  for (int f = threadIdx.x; f < P.nfac; f += kThreads) area[f] = mem[M.area + f];
   if (threadIdx.x == 0) eval_blmatrix(x[P.ncoef], x[P.ncoef + 1], blm);
   __syncthreads();
+  (void)g;
   if constexpr (F32)
     model_tile_f32<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
   else
     model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// model_mfma_kernel: one workgroup per (slot, tile of kTilePts points);
+// blockIdx.x = tile * nslots + slot.  The same model as model_tile
+// (period_search/bright.c:14-126) with the shape-derivative contraction on
+// the matrix cores:
+//   facet stage (VALU, FP64): thread (point pl, sub-slot s) evaluates the
+//     facets f = f0 + s + 4i of each chunk: visibility, scattering law,
+//     the weight W[p][f] = Area_f s_pf (0 if not visible) into an LDS tile,
+//     and its partial sums of br, V, U, cl- and cls-derivatives;
+//   contraction (MFMA, FP64): wave w owns points 16w..16w+15 and computes
+//     S[p][c] = sum_f W[p][f] Dsph[f][c] with v_mfma_f64_16x16x4 over the
+//     chunk's Dsph rows staged in LDS (A = W, B = Dsph, NB blocks of 16
+//     columns).  D layout: lane holds D[4r + lane/16][lane%16], r = 0..3.
+// LDS: x[kMaxMa] blm[32] area[F] geom[kTilePts] W[kFacetChunk][kTilePts]
+//      Dsph chunk[kFacetChunk][16 NB]; the last two regions are reused for
+//      the partial-sum reduction at the end.
+template <int NB>
+__global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P, DevWork W) {
+  extern __shared__ double sm[];
+  const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
+  if (!W.st[slot].eval) return;
+  constexpr int NC = 16 * NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  const int F = P.nfac;
+  double *x = sm, *blm = x + kMaxMa, *area = blm + 32;
+  PointGeom *gst = reinterpret_cast<PointGeom *>(area + F);
+  double *wt = reinterpret_cast<double *>(gst + kTilePts);
+  double *dch = wt + kFacetChunk * kTilePts;
+  double *red = wt;  // [4][kTilePts][9] after the last chunk
+
+  for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
+  for (int f = tid; f < F; f += kThreads) area[f] = mem[M.area + f];
+  if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
+  __syncthreads();
+  const int p0 = tile * kTilePts;
+  if (tid < kTilePts) {
+    const int p = min(p0 + tid, P.n_points - 1);
+    PointGeom G;
+    point_geometry(P, p, x, blm, G);
+    gst[tid] = G;
+  }
+  __syncthreads();
+  const int pl = tid % kTilePts, sub = tid / kTilePts;
+  const double e_0 = gst[pl].e[0], e_1 = gst[pl].e[1], e_2 = gst[pl].e[2];
+  const double s_0 = gst[pl].e0[0], s_1 = gst[pl].e0[1], s_2 = gst[pl].e0[2];
+  const double cl = exp(x[P.ma - 2]);
+  const double cls = x[P.ma - 1];
+  double br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
+  v4d acc[NB];
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) acc[cb] = v4d{0, 0, 0, 0};
+  const double4 *__restrict__ n4 = P.nor4;
+  const double *__restrict__ dsp = P.dsph;
+  const int ncp = P.ncp;
+  for (int f0 = 0; f0 < F; f0 += kFacetChunk) {
+    // stage the chunk's Dsph rows (zero beyond F and beyond ncp)
+    for (int idx = tid; idx < kFacetChunk * NC; idx += kThreads) {
+      const int r = idx / NC, c = idx - r * NC, f = f0 + r;
+      dch[idx] = (f < F && c < ncp) ? dsp[(size_t)f * ncp + c] : 0.0;
+    }
+    // facet stage
+#pragma unroll 2
+    for (int i = 0; i < kFacetChunk / 4; i++) {
+      const int fl = sub + 4 * i, f = f0 + fl;
+      double wv = 0.0;
+      if (f < F) {
+        const double4 nv = n4[f];
+        const double mu = e_0 * nv.x + e_1 * nv.y + e_2 * nv.z;
+        const double mu0 = s_0 * nv.x + s_1 * nv.y + s_2 * nv.z;
+        if (mu > kTiny && mu0 > kTiny) {
+          const double af = area[f];
+          const double rd = recip(mu + mu0);
+          const double mm = mu * mu0;
+          wv = af * (mm * (cl + cls * rd));
+          br += wv;
+          const double q0 = mu0 * rd, q1 = mu * rd;
+          const double ad = af * (cls * q0 * q0 + cl * mu0);
+          const double ad0 = af * (cls * q1 * q1 + cl * mu);
+          v0 += ad * nv.x;
+          v1 += ad * nv.y;
+          v2 += ad * nv.z;
+          u0 += ad0 * nv.x;
+          u1 += ad0 * nv.y;
+          u2 += ad0 * nv.z;
+          const double am = af * mm;
+          scl += am;
+          scls += am * rd;
+        }
+      }
+      wt[fl * kTilePts + pl] = wv;
+    }
+    __syncthreads();
+    // contraction on the matrix cores: wave = row block of 16 points
+#pragma unroll
+    for (int kk = 0; kk < kFacetChunk / 4; kk++) {
+      const int kr = 4 * kk + (lane >> 4);
+      const double av = wt[kr * kTilePts + 16 * wave + (lane & 15)];
+#pragma unroll
+      for (int cb = 0; cb < NB; cb++) {
+        const double bv = dch[kr * NC + 16 * cb + (lane & 15)];
+        acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // reduce the per-point partial sums of the 4 sub-slots
+  {
+    double *rp = red + ((size_t)sub * kTilePts + pl) * 9;
+    rp[0] = br;
+    rp[1] = v0;
+    rp[2] = v1;
+    rp[3] = v2;
+    rp[4] = u0;
+    rp[5] = u1;
+    rp[6] = u2;
+    rp[7] = scl;
+    rp[8] = scls;
+  }
+  __syncthreads();
+  const int ld = P.npad;
+  double *J = mem + M.J;
+  if (tid < kTilePts && p0 + tid < P.n_points) {
+    const int p = p0 + tid;
+    double t[9];
+    for (int k = 0; k < 9; k++) t[k] = 0;
+    for (int q = 0; q < 4; q++)
+      for (int k = 0; k < 9; k++) t[k] += red[((size_t)q * kTilePts + tid) * 9 + k];
+    const PointGeom G = gst[tid];
+    const double sc = G.scale;
+    const double V[3] = {t[1], t[2], t[3]}, U[3] = {t[4], t[5], t[6]};
+    double r[3];
+    rotation_derivs(P, p, blm, G, V, U, r);
+    for (int k = 0; k < 3; k++) J[(size_t)(P.ncoef + k) * ld + p] = sc * r[k];
+    for (int i = 0; i < 3; i++) J[(size_t)(P.ncoef + 3 + i) * ld + p] = t[0] * G.dphp[i];
+    J[(size_t)(P.ncoef + 6) * ld + p] = sc * t[7] * cl;
+    J[(size_t)(P.ncoef + 7) * ld + p] = sc * t[8];
+    J[(size_t)P.ma * ld + p] = t[0] * sc;
+  }
+  // shape derivatives from the accumulators
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) {
+    const int c = 16 * cb + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int pp = 16 * wave + 4 * r + (lane >> 4);
+      const int p = p0 + pp;
+      if (p < P.n_points && c < P.ncoef) J[(size_t)c * ld + p] = gst[pp].scale * acc[cb][r];
+    }
+  }
 }
 
 // neq_kernel: one workgroup per slot.  Convexity rows, normal equations at
@@ -855,8 +1038,10 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) 
   if (eval) {
     if (tid == 0) atomicAdd(W.done + 1, 1);
     for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
-    __syncthreads();
-    eval_curv(P, x, area, g);
+    for (int f = tid; f < P.nfac; f += kThreads) {
+      area[f] = mem[M.area + f];
+      g[f] = mem[M.g + f];
+    }
     __syncthreads();
     conv_rows(P, area, g, mem + M.J, scal + 1);
     __threadfence_block();
@@ -1031,6 +1216,28 @@ ModelFn pick_model(int ncp) {
   }
 }
 
+using MfmaFn = void (*)(DevProblem, DevWork);
+MfmaFn pick_mfma(int ncoef) {
+  switch ((ncoef + 15) / 16) {
+    case 1: return model_mfma_kernel<1>;
+    case 2: return model_mfma_kernel<2>;
+    case 3: return model_mfma_kernel<3>;
+    case 4: return model_mfma_kernel<4>;
+    case 5: return model_mfma_kernel<5>;
+    case 6: return model_mfma_kernel<6>;
+    case 7: return model_mfma_kernel<7>;
+    case 8: return model_mfma_kernel<8>;
+    default: return nullptr;
+  }
+}
+size_t mfma_lds(int nfac, int ncoef) {
+  const int nc = (ncoef + 15) / 16 * 16;
+  size_t d = (size_t)kMaxMa + 32 + nfac + (size_t)kFacetChunk * kTilePts + (size_t)kFacetChunk * nc;
+  size_t red = 4 * kTilePts * 9;
+  if ((size_t)kFacetChunk * kTilePts + (size_t)kFacetChunk * nc < red) d += red;
+  return d * sizeof(double) + kTilePts * sizeof(PointGeom);
+}
+
 size_t step_lds(int mfit, int nfac) {
   return ((size_t)mfit * mfit + 3 * kMaxMfit + 2 * kWaves + 2 * (size_t)nfac + kMaxMfit / 2 + 2) * sizeof(double);
 }
@@ -1055,6 +1262,7 @@ struct ps_engine {
   int n_poles = 0, cu_count = 0, max_slots = 2048;
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
   ModelFn model = nullptr;
+  bool use_mfma = true;   // FP64: matrix-core contraction (default) or the VALU kernel
   // slot memory
   SlotState *st = nullptr;
   double *mem = nullptr;
@@ -1171,7 +1379,8 @@ int run_batch(ps_engine *e, DevWork W) {
     for (int k = 0; k < poll; k++) {
       hipLaunchKernelGGL(step_kernel, dim3(nslots), dim3(kThreads), e->lds_step, e->stream, P, W);
       if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k], e->stream));
-      hipLaunchKernelGGL(e->model, dim3(nslots * P.ntiles), dim3(kThreads), e->lds_model, e->stream, P, W);
+      hipLaunchKernelGGL(e->model, dim3(nslots * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads),
+                         e->lds_model, e->stream, P, W);
       if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k + 1], e->stream));
       hipLaunchKernelGGL(neq_kernel, dim3(nslots), dim3(kThreads), e->lds_neq, e->stream, P, W);
     }
@@ -1327,7 +1536,10 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
     if (ia[l]) fidx.push_back(l);
   const int mfit = (int)fidx.size();
   const int ncp = (ncoef + 7) / 8 * 8;
+  const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs)
+  e->use_mfma = e->precision == PS_PREC_FP64 && !(kv && !strcmp(kv, "valu"));
   ModelFn mk = e->precision == PS_PREC_FP32 ? pick_model<true>(ncp) : pick_model<false>(ncp);
+  if (e->use_mfma) mk = pick_mfma(ncoef);
   if (!mk) {
     e->err = "no model kernel for this coefficient count";
     return PS_ERR_LIMIT;
@@ -1352,11 +1564,13 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   lc_inrel[p->n_lc] = 0;
   std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
   std::vector<float> nxf(F), nyf(F), nzf(F), dsphf((size_t)F * ncp, 0.0f);
+  std::vector<double4> nor4(F);
   for (int f = 0; f < F; f++) {
     nx[f] = p->nor[3 * f];
     ny[f] = p->nor[3 * f + 1];
     nz[f] = p->nor[3 * f + 2];
     for (int c = 0; c < ncoef; c++) dsph[(size_t)f * ncp + c] = p->dsph[(size_t)f * ncoef + c];
+    nor4[f] = make_double4(nx[f], ny[f], nz[f], 0.0);
     nxf[f] = (float)nx[f];
     nyf[f] = (float)ny[f];
     nzf[f] = (float)nz[f];
@@ -1395,6 +1609,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   UP(nz.data(), F, nz);
   UP(p->darea, F, darea);
   UP(dsph.data(), dsph.size(), dsph);
+  UP(nor4.data(), F, nor4);
   UP(nxf.data(), F, nxf);
   UP(nyf.data(), F, nyf);
   UP(nzf.data(), F, nzf);
@@ -1439,10 +1654,11 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   D.ndata = N + 3;
   D.npad = (N + 3 + 63) / 64 * 64;
   D.ntiles = (N + kThreads - 1) / kThreads;
+  D.ntiles64 = (N + kTilePts - 1) / kTilePts;
   e->n_poles = p->n_poles;
   e->model = mk;
   e->lds_step = step_lds(mfit, F);
-  e->lds_model = model_lds(F);
+  e->lds_model = e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F);
   e->lds_neq = neq_lds(mfit, F);
   const size_t lds_max = 160 * 1024;
   if (e->lds_step > lds_max || e->lds_neq > lds_max || e->lds_model > lds_max) {

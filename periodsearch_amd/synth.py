@@ -40,13 +40,17 @@ def coef_degrees(lmax: int, mmax: int) -> np.ndarray:
     return np.array(out)
 
 
-def orbit_vectors(jd: np.ndarray, a_ast: float = 2.5, phase0: float = 0.7):
-    """Asteroid-centric unit-free vectors to the Sun and to the Earth (AU)."""
+def orbit_vectors(jd: np.ndarray, a_ast: float = 2.5, phase0: float = 0.7, incl_deg: float = 0.0):
+    """Asteroid-centric vectors to the Sun and to the Earth (AU).  The
+    asteroid's circular orbit is inclined by incl_deg to the ecliptic (0 =
+    coplanar; then every aspect lies in the ecliptic and the poles lambda and
+    lambda+180 deg fit the data equally well)."""
     t = jd - JD_EPOCH
     th_e = 2 * np.pi * t / YEAR
     th_a = 2 * np.pi * t / (YEAR * a_ast ** 1.5) + phase0
     earth = np.stack([np.cos(th_e), np.sin(th_e), np.zeros_like(t)], 1)
-    ast = a_ast * np.stack([np.cos(th_a), np.sin(th_a), np.zeros_like(t)], 1)
+    ci, si = np.cos(np.radians(incl_deg)), np.sin(np.radians(incl_deg))
+    ast = a_ast * np.stack([np.cos(th_a), np.sin(th_a) * ci, np.sin(th_a) * si], 1)
     return -ast, earth - ast
 
 
@@ -80,26 +84,26 @@ def model_brightness(cg: np.ndarray, basis: dict, ee: np.ndarray, ee0: np.ndarra
     return scale * br
 
 
-def _observable(jd: np.ndarray) -> np.ndarray:
+def _observable(jd: np.ndarray, incl_deg: float = 0.0) -> np.ndarray:
     """Solar elongation of the asteroid seen from the Earth above 60 deg."""
-    sun, earth = orbit_vectors(jd)
+    sun, earth = orbit_vectors(jd, incl_deg=incl_deg)
     to_ast = -earth
     to_sun = sun - earth
     cosel = np.sum(to_ast * to_sun, 1) / (np.linalg.norm(to_ast, axis=1) * np.linalg.norm(to_sun, axis=1))
     return cosel < 0.5
 
 
-def _epochs(rng, n: int, span_days: float) -> np.ndarray:
+def _epochs(rng, n: int, span_days: float, incl_deg: float = 0.0) -> np.ndarray:
     out = []
     while len(out) < n:
         jd = JD_EPOCH + rng.uniform(0, span_days, size=4 * n)
-        out.extend(jd[_observable(jd)].tolist())
+        out.extend(jd[_observable(jd, incl_deg)].tolist())
     return np.array(out[:n])
 
 
 def make_asteroid(seed: int, *, lmax: int, mmax: int, nrows: int, n_dense: int, dense_pts: int, n_sparse: int,
                   noise: float, span_years: float, period_h: float, pole=(120.0, 40.0), cadence_min: float = 5.0,
-                  sparse_lc_max: int = 1000) -> dict:
+                  sparse_lc_max: int = 1000, incl_deg: float = 0.0) -> dict:
     """Observations of one synthetic asteroid (no file written)."""
     from . import shape_basis
 
@@ -113,10 +117,10 @@ def make_asteroid(seed: int, *, lmax: int, mmax: int, nrows: int, n_dense: int, 
     span = span_years * YEAR
     lcs = []
     for _ in range(n_dense):
-        t0 = _epochs(rng, 1, span - 1.0)[0]
+        t0 = _epochs(rng, 1, span - 1.0, incl_deg)[0]
         lcs.append(t0 + np.arange(dense_pts) * cadence_min / 1440.0)
     if n_sparse:
-        sp = np.sort(_epochs(rng, n_sparse, span))
+        sp = np.sort(_epochs(rng, n_sparse, span, incl_deg))
         for k in range(0, n_sparse, sparse_lc_max):
             lcs.append(sp[k:k + sparse_lc_max])
     jd_all = np.concatenate(lcs)
@@ -125,7 +129,7 @@ def make_asteroid(seed: int, *, lmax: int, mmax: int, nrows: int, n_dense: int, 
     omega = 24 * 2 * np.pi / period_h
     out_lcs = []
     for jd in lcs:
-        sun, earth = orbit_vectors(jd)
+        sun, earth = orbit_vectors(jd, incl_deg=incl_deg)
         ee = earth / np.linalg.norm(earth, axis=1)[:, None]
         ee0 = sun / np.linalg.norm(sun, axis=1)[:, None]
         y = model_brightness(cg, basis, ee, ee0, jd - jd0, beta, lam, omega)
