@@ -52,6 +52,8 @@ constexpr int kMaxFacets = 1000;
 constexpr double kTiny = 1e-8;     // period_search/constants.h:10
 constexpr double kPi = 3.14159265358979323846;
 
+typedef double v4d __attribute__((ext_vector_type(4)));  // one 16x16 FP64 MFMA accumulator per lane
+
 #define HIP_TRY(x)                                                         \
   do {                                                                     \
     hipError_t err_ = (x);                                                 \
@@ -571,6 +573,144 @@ __device__ void normal_equations(const DevProblem &P, const double *J, double co
   if (tid == kThreads - 1) *chi_out = chi;
 }
 
+// Pass 2 on the matrix cores: relative renormalisation (mrqcof.c:42-104)
+// and the normal equations (mrqcof.c:106-139) with alpha = WT^T Jn summed by
+// v_mfma_f64_16x16x4 over blocks of 4 points.  Points are staged kNeqChunk
+// at a time in LDS as renormalised rows Jn of the free parameters (columns
+// padded with zeros to MP = 16*nb, row stride MP+1); the weighted row
+// WT = Jn*sig2i*w is formed in registers with the reference's operations.
+// The lower-triangle 16x16 tiles (jb >= kb) are spread over the 4 waves
+// (tile t on wave t%4, at most NT per wave); alpha takes the j >= k entries
+// and mirrors them, as the reference does.  beta and chi2 are summed by
+// single threads in point order.
+constexpr int kNeqChunk = 64;
+
+template <int NT>
+__device__ void normal_equations_mfma(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
+                                      double *beta, double *chi_out, double *alpha) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ld = P.npad, mfit = P.mfit, ma = P.ma;
+  const int nb = (mfit + 15) / 16, MP = 16 * nb, MS = MP + 1;
+  const int ntile = nb * (nb + 1) / 2;
+  v4d acc[NT];
+  int tj[NT], tk[NT];
+#pragma unroll
+  for (int i = 0; i < NT; i++) {
+    acc[i] = v4d{0, 0, 0, 0};
+    int t = wave + 4 * i, jb = 0;
+    while ((jb + 1) * (jb + 2) / 2 <= t) jb++;
+    tj[i] = jb;
+    tk[i] = t - jb * (jb + 1) / 2;
+  }
+  double *Jn = wst;                        // [kNeqChunk][MS]
+  double *aux = wst + kNeqChunk * MS;      // [kNeqChunk][3]: sig2i, weight, dy
+  double bacc = 0, chi = 0;
+  for (int lc = 0; lc <= P.n_lc; lc++) {
+    const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
+    const int inrel = P.lc_inrel[lc];
+    if (inrel) {
+      // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda
+      for (int col = tid; col <= ma; col += kThreads) {
+        const double *cp = J + (size_t)col * ld;
+        double sum = 0;
+        for (int q = q0; q < q1; q++) sum = sum + cp[q];
+        dave[col] = sum;
+      }
+      __syncthreads();
+    }
+    for (int c0 = q0; c0 < q1; c0 += kNeqChunk) {
+      const int cn = min(kNeqChunk, q1 - c0);
+      const int cn4 = (cn + 3) & ~3;
+      for (int pp = tid; pp < cn4; pp += kThreads) {
+        double s2 = 0, wg = 0, dy = 0;
+        if (pp < cn) {
+          const int q = c0 + pp;
+          double ymod = J[(size_t)ma * ld + q];
+          double sg, yo;
+          if (q < P.n_points) {
+            sg = P.sig[q];
+            wg = P.w[q];
+            yo = P.y[q];
+          } else {
+            sg = 1 / conw_r;
+            wg = 1;
+            yo = 0;
+          }
+          if (inrel) {
+            const double ave = dave[ma];
+            const double coef = sg * np / ave;
+            ymod = coef * ymod;
+          }
+          s2 = 1 / (sg * sg);
+          dy = yo - ymod;
+        }
+        aux[pp * 3 + 0] = s2;
+        aux[pp * 3 + 1] = wg;
+        aux[pp * 3 + 2] = dy;
+      }
+      // renormalised rows, coalesced over points
+      for (int idx = tid; idx < MP * cn4; idx += kThreads) {
+        const int jf = idx / cn4, pp = idx - jf * cn4;
+        double d = 0.0;
+        if (jf < mfit && pp < cn) {
+          const int q = c0 + pp;
+          const int l = P.fidx[jf];
+          d = J[(size_t)l * ld + q];
+          if (inrel) {
+            const double ave = dave[ma];
+            const double coef = (q < P.n_points ? P.sig[q] : 1 / conw_r) * np / ave;
+            d = (l == 0) ? 0.0 : coef * (d - J[(size_t)ma * ld + q] * dave[l] / ave);
+          }
+        }
+        Jn[pp * MS + jf] = d;
+      }
+      __syncthreads();
+      for (int kk = 0; kk < cn4 / 4; kk++) {
+        const int row = 4 * kk + (lane >> 4);
+        const double s2 = aux[row * 3], wg = aux[row * 3 + 1];
+#pragma unroll
+        for (int i = 0; i < NT; i++) {
+          if (wave + 4 * i < ntile) {
+            const double a = Jn[row * MS + 16 * tj[i] + (lane & 15)] * s2 * wg;
+            const double b = Jn[row * MS + 16 * tk[i] + (lane & 15)];
+            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+          }
+        }
+      }
+      if (tid < mfit) {
+        for (int pp = 0; pp < cn; pp++) {
+          const double wt = Jn[pp * MS + tid] * aux[pp * 3] * aux[pp * 3 + 1];
+          bacc = bacc + aux[pp * 3 + 2] * wt;
+        }
+      }
+      if (tid == kThreads - 1) {
+        for (int pp = 0; pp < cn; pp++) {
+          const double dy = aux[pp * 3 + 2];
+          chi = chi + dy * dy * aux[pp * 3] * aux[pp * 3 + 1];
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NT; i++) {
+    if (wave + 4 * i < ntile) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = 16 * tj[i] + 4 * r + (lane >> 4);
+        const int k = 16 * tk[i] + (lane & 15);
+        if (j < mfit && k < mfit && j >= k) {
+          alpha[j * mfit + k] = acc[i][r];
+          alpha[k * mfit + j] = acc[i][r];
+        }
+      }
+    }
+  }
+  if (tid < mfit) beta[tid] = bacc;
+  if (tid == kThreads - 1) *chi_out = chi;
+}
+
 // Gauss-Jordan elimination with full pivoting on the LDS matrix A (n x n)
 // and right-hand side b (period_search/gauss_errc.c:9-73).  Only the
 // solution b is needed by the caller, so columns that are already pivoted
@@ -863,8 +1003,6 @@ __global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(Dev
     model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
 }
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
 // model_mfma_kernel: one workgroup per (slot, tile of kTilePts points);
 // blockIdx.x = tile * nslots + slot.  The same model as model_tile
 // (period_search/bright.c:14-126) with the shape-derivative contraction on
@@ -1021,6 +1159,7 @@ __global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P, DevW
 // the trial parameters, then accept/reject and the driver bookkeeping.
 // LDS: staging [2*kChunk*mfit + 3*kChunk], area/g [F], dave [ma+1], x [ma],
 // scalars.
+template <int NT>
 __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
@@ -1030,7 +1169,8 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) 
   const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
   const SlotMem M = slot_mem(ma, mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
-  double *wst = sm, *area = wst + 2 * kChunk * mfit + 3 * kChunk, *g = area + P.nfac, *dave = g + P.nfac;
+  const int MS = (mfit + 15) / 16 * 16 + 1;
+  double *wst = sm, *area = wst + (size_t)kNeqChunk * (MS + 3), *g = area + P.nfac, *dave = g + P.nfac;
   double *x = dave + kMaxMa + 1, *scal = x + kMaxMa;  // scal[0] = chi2, [1..3] = chck
   const int eval = S.eval;
   const int sel = S.sel;
@@ -1046,7 +1186,7 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) 
     conv_rows(P, area, g, mem + M.J, scal + 1);
     __threadfence_block();
     __syncthreads();
-    normal_equations(P, mem + M.J, W.conw_r, wst, dave, btr, scal, atr);
+    normal_equations_mfma<NT>(P, mem + M.J, W.conw_r, wst, dave, btr, scal, atr);
     __threadfence_block();
     __syncthreads();
   }
@@ -1245,7 +1385,18 @@ size_t model_lds(int nfac) {
   return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
 }
 size_t neq_lds(int mfit, int nfac) {
-  return ((size_t)2 * kChunk * mfit + 3 * kChunk + 2 * (size_t)nfac + 2 * kMaxMa + 1 + 8) * sizeof(double);
+  const int MS = (mfit + 15) / 16 * 16 + 1;
+  return ((size_t)kNeqChunk * (MS + 3) + 2 * (size_t)nfac + 2 * kMaxMa + 1 + 8) * sizeof(double);
+}
+
+using NeqFn = void (*)(DevProblem, DevWork);
+NeqFn pick_neq(int mfit) {
+  const int nb = (mfit + 15) / 16, ntile = nb * (nb + 1) / 2, nt = (ntile + 3) / 4;
+  if (nt <= 1) return neq_kernel<1>;
+  if (nt <= 3) return neq_kernel<3>;
+  if (nt <= 6) return neq_kernel<6>;
+  if (nt <= 9) return neq_kernel<9>;
+  return nullptr;
 }
 
 }  // namespace
@@ -1262,6 +1413,7 @@ struct ps_engine {
   int n_poles = 0, cu_count = 0, max_slots = 2048;
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
   ModelFn model = nullptr;
+  NeqFn neq = nullptr;
   bool use_mfma = true;   // FP64: matrix-core contraction (default) or the VALU kernel
   // slot memory
   SlotState *st = nullptr;
@@ -1382,7 +1534,7 @@ int run_batch(ps_engine *e, DevWork W) {
       hipLaunchKernelGGL(e->model, dim3(nslots * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads),
                          e->lds_model, e->stream, P, W);
       if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k + 1], e->stream));
-      hipLaunchKernelGGL(neq_kernel, dim3(nslots), dim3(kThreads), e->lds_neq, e->stream, P, W);
+      hipLaunchKernelGGL(e->neq, dim3(nslots), dim3(kThreads), e->lds_neq, e->stream, P, W);
     }
     HIP_TRY(hipGetLastError());
     it += poll;
@@ -1667,7 +1819,12 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   }
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(step_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_step));
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(neq_kernel),
+  e->neq = pick_neq(mfit);
+  if (!e->neq) {
+    e->err = "too many free parameters";
+    return PS_ERR_LIMIT;
+  }
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(e->neq),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_neq));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(mk), hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)e->lds_model));
