@@ -610,13 +610,28 @@ __device__ void normal_equations_mfma(const DevProblem &P, const double *J, doub
     const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
     const int inrel = P.lc_inrel[lc];
     if (inrel) {
-      // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda
-      for (int col = tid; col <= ma; col += kThreads) {
+      // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda.
+      // Each column is summed by 4 threads over interleaved quarters of the
+      // points (8 independent loads in flight each), then combined in order.
+      double *part = Jn;  // [4][ma+1] scratch, free before the chunk loop
+      for (int cs = tid; cs < 4 * (ma + 1); cs += kThreads) {
+        const int col = cs % (ma + 1), seg = cs / (ma + 1);
         const double *cp = J + (size_t)col * ld;
+        double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int q = q0 + seg;
+        for (; q + 28 < q1; q += 32) {
+#pragma unroll
+          for (int u = 0; u < 8; u++) s8[u] = s8[u] + cp[q + 4 * u];
+        }
+        for (int u = 0; q < q1; q += 4, u++) s8[u & 7] = s8[u & 7] + cp[q];
         double sum = 0;
-        for (int q = q0; q < q1; q++) sum = sum + cp[q];
-        dave[col] = sum;
+#pragma unroll
+        for (int u = 0; u < 8; u++) sum = sum + s8[u];
+        part[seg * (ma + 1) + col] = sum;
       }
+      __syncthreads();
+      for (int col = tid; col <= ma; col += kThreads)
+        dave[col] = ((part[col] + part[(ma + 1) + col]) + part[2 * (ma + 1) + col]) + part[3 * (ma + 1) + col];
       __syncthreads();
     }
     for (int c0 = q0; c0 < q1; c0 += kNeqChunk) {

@@ -179,6 +179,9 @@ def write_workload(name: str, path: str, **overrides) -> dict:
     cfg = dict(WORKLOADS[name])
     cfg.update(overrides)
     per_start, per_end = cfg.pop("per_start"), cfg.pop("per_end")
+    per_start, per_end = overrides.get("per_start", per_start), overrides.get("per_end", per_end)
+    cfg.pop("per_start", None)
+    cfg.pop("per_end", None)
     stop = cfg.pop("stop", 50)
     ast = make_asteroid(cfg.pop("seed"), **cfg)
     write_input(path, ast, per_start, per_end, stop=stop)

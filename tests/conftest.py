@@ -21,9 +21,21 @@ def golden_dir():
 
 @pytest.fixture(scope="session")
 def small_input(tmp_path_factory):
-    """The 400-point synthetic asteroid (lmax=mmax=6, nrows=6, 7.0-7.6 h)."""
+    """The 400-point synthetic asteroid (lmax=mmax=6, nrows=6, 7.0-7.6 h)
+    on a coplanar orbit (every aspect in the ecliptic)."""
     from periodsearch_amd import synth
 
     p = str(tmp_path_factory.mktemp("small") / "period_search_in")
     synth.write_workload("small", p)
+    return p
+
+
+@pytest.fixture(scope="session")
+def small_incl_input(tmp_path_factory):
+    """The same asteroid on an orbit inclined by 10 deg to the ecliptic, so
+    that the pole is not ambiguous between lambda and lambda+180 deg."""
+    from periodsearch_amd import synth
+
+    p = str(tmp_path_factory.mktemp("small_incl") / "period_search_in")
+    synth.write_workload("small", p, incl_deg=10.0)
     return p

@@ -20,7 +20,7 @@ def short_scan(tmp_path_factory):
     # 6 trial periods: 1/P_end = 1/7 - 5 * freq_step (8-yr span, step coef 0.5);
     # lmax=mmax=3 keeps the oracle's CPU run short
     cfg = {k: v for k, v in synth.WORKLOADS["small"].items() if k not in ("seed", "per_start", "per_end")}
-    cfg.update(lmax=3, mmax=3)
+    cfg.update(lmax=3, mmax=3, incl_deg=10.0)
     ast = synth.make_asteroid(0, **cfg)
     synth.write_input(path, ast, 7.0, 7.0009)
     return str(d), path
