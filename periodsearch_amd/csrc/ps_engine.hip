@@ -104,19 +104,20 @@ enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_DRAINED = 3 };
 
 // Offsets (in doubles) inside one slot's memory block.
 struct SlotMem {
-  long long J, alpha0, alpha1, beta0, beta1, a, atry, alast, area, g, total;
+  long long J, alpha0, alpha1, beta0, beta1, da, a, atry, alast, area, g, total;
 };
 
 __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
   SlotMem m;
   long long o = 0;
-  m.area = o; o += kMaxFacets;   // curv() at atry: Area, g (written by step_kernel)
+  m.area = o; o += kMaxFacets;   // curv() at atry: Area, g (written by curv_kernel)
   m.g = o; o += kMaxFacets;
   m.J = o; o += (long long)(ma + 1) * npad;
   m.alpha0 = o; o += (long long)mfit * mfit;
   m.alpha1 = o; o += (long long)mfit * mfit;
   m.beta0 = o; o += kMaxMfit;
   m.beta1 = o; o += kMaxMfit;
+  m.da = o; o += kMaxMfit;
   m.a = o; o += kMaxMa;
   m.atry = o; o += kMaxMa;
   m.alast = o; o += kMaxMa;
@@ -140,24 +141,29 @@ struct DevWork {
   long long slot_doubles;
 };
 
-// curv (period_search/curv.c:10-38): Area and g of every facet for the
-// shape part of a.  Same operation order as the reference.
+// curv (period_search/curv.c:10-38): g of facet f for the shape part of a
+// (Area = Darea * g).  Same operation order as the reference.
+__device__ inline double curv_g(const DevProblem &P, const double *a, int f) {
+#pragma clang fp contract(off)
+  double s = 0;
+  int n = 0;
+  for (int q = 0; q < P.ngrp; q++) {
+    const int m = P.grp_m[q];
+    double fsum = a[n] * P.cfc[m * P.nfac + f];
+    n++;
+    if (m != 0) {
+      fsum = fsum + a[n] * P.cfs[m * P.nfac + f];
+      n++;
+    }
+    s = s + P.cpleg[q * P.nfac + f] * fsum;
+  }
+  return exp(s);
+}
+
 __device__ void eval_curv(const DevProblem &P, const double *a, double *area, double *g) {
 #pragma clang fp contract(off)
-  for (int f = threadIdx.x; f < P.nfac; f += kThreads) {
-    double s = 0;
-    int n = 0;
-    for (int q = 0; q < P.ngrp; q++) {
-      int m = P.grp_m[q];
-      double fsum = a[n] * P.cfc[m * P.nfac + f];
-      n++;
-      if (m != 0) {
-        fsum = fsum + a[n] * P.cfs[m * P.nfac + f];
-        n++;
-      }
-      s = s + P.cpleg[q * P.nfac + f] * fsum;
-    }
-    double gf = exp(s);
+  for (int f = threadIdx.x; f < P.nfac; f += blockDim.x) {
+    const double gf = curv_g(P, a, f);
     g[f] = gf;
     area[f] = P.darea[f] * gf;
   }
@@ -726,125 +732,13 @@ __device__ void normal_equations_mfma(const DevProblem &P, const double *J, doub
   if (tid == kThreads - 1) *chi_out = chi;
 }
 
-// Gauss-Jordan elimination with full pivoting on the LDS matrix A (n x n)
-// and right-hand side b (period_search/gauss_errc.c:9-73).  Only the
-// solution b is needed by the caller, so columns that are already pivoted
-// are not updated; every value that the reference's b depends on is
-// computed with the reference's operations in the reference's order.
-// Returns 0, 1 (singular) or 2 (zero pivot), uniform across the workgroup.
-__device__ int gauss_solve(double *A, int n, double *b, int *ipiv, double *prow, double *dumv,
-                                        double *red) {
-#pragma clang fp contract(off)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int j = tid; j < n; j += kThreads) ipiv[j] = 0;
-  __syncthreads();
-  int prev_irow = -1, prev_icol = -1;
-  for (int i = 0; i < n; i++) {
-    // 1. argmax |A[j][k]| over unpivoted rows/columns; ties keep the last
-    //    entry in row-major order (fabs(a) >= big in the reference).
-    double bv = -1.0;
-    int be = -1;
-    for (int e = tid; e < n * n; e += kThreads) {
-      const int j = e / n, k = e - j * n;
-      if (ipiv[j] != 1 && ipiv[k] == 0) {
-        const double v = fabs(A[e]);
-        if (v >= bv) {
-          bv = v;
-          be = e;
-        }
-      }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      double ov = __shfl_xor(bv, o, 64);
-      int oe = __shfl_xor(be, o, 64);
-      if (ov > bv || (ov == bv && oe > be)) {
-        bv = ov;
-        be = oe;
-      }
-    }
-    if (lane == 0) {
-      red[2 * wave] = bv;
-      red[2 * wave + 1] = (double)be;
-    }
-    __syncthreads();
-    bv = red[0];
-    be = (int)red[1];
-    for (int w = 1; w < kWaves; w++) {
-      double ov = red[2 * w];
-      int oe = (int)red[2 * w + 1];
-      if (ov > bv || (ov == bv && oe > be)) {
-        bv = ov;
-        be = oe;
-      }
-    }
-    int irow, icol;
-    if (be < 0) {  // no candidate (NaN matrix): the reference keeps the previous pivot
-      if (prev_icol < 0) return 1;
-      irow = prev_irow;
-      icol = prev_icol;
-    } else {
-      irow = be / n;
-      icol = be - irow * n;
-    }
-    if (ipiv[icol] != 0) return 1;  // column pivoted twice: the next scan reports singular
-    prev_irow = irow;
-    prev_icol = icol;
-    // 2. swap rows irow and icol in the columns that are still live
-    if (irow != icol) {
-      for (int l = tid; l < n; l += kThreads) {
-        if (ipiv[l] == 0) {
-          double t = A[irow * n + l];
-          A[irow * n + l] = A[icol * n + l];
-          A[icol * n + l] = t;
-        }
-      }
-      if (tid == 0) {
-        double t = b[irow];
-        b[irow] = b[icol];
-        b[icol] = t;
-      }
-    }
-    __syncthreads();
-    // 3. pivot row scaled by 1/pivot, multipliers of the other rows
-    const double piv = A[icol * n + icol];
-    if (piv == 0.0) return 2;
-    const double pivinv = 1.0 / piv;
-    for (int l = tid; l < n; l += kThreads) {
-      prow[l] = (ipiv[l] == 0 && l != icol) ? A[icol * n + l] * pivinv : 0.0;
-      dumv[l] = A[l * n + icol];
-    }
-    __syncthreads();
-    const double bpiv = b[icol] * pivinv;
-    // 4. eliminate
-    for (int e = tid; e < n * n; e += kThreads) {
-      const int ll = e / n, l = e - ll * n;
-      if (ipiv[l] == 0 && l != icol) {
-        if (ll == icol)
-          A[e] = prow[l];
-        else
-          A[e] = A[e] - prow[l] * dumv[ll];
-      }
-    }
-    for (int ll = tid; ll < n; ll += kThreads) {
-      if (ll != icol) b[ll] = b[ll] - bpiv * dumv[ll];
-    }
-    __syncthreads();
-    if (tid == 0) {
-      b[icol] = bpiv;
-      ipiv[icol] = 1;
-    }
-    __syncthreads();
-  }
-  return 0;
-}
-
 // Fit initialisation (period_search_BOINC.cpp:819-844): shape from the
 // ellipsoid, pole m, omega from the trial frequency, phase and scattering
 // parameters.
 __device__ void init_params(const DevProblem &P, double freq, int m, double *a) {
 #pragma clang fp contract(off)
   const int nc = P.ncoef;
-  for (int l = threadIdx.x; l < P.ma; l += kThreads) {
+  for (int l = threadIdx.x; l < P.ma; l += blockDim.x) {
     double v;
     if (l < nc)
       v = P.cg_first[l];
@@ -869,19 +763,131 @@ __device__ void init_params(const DevProblem &P, double freq, int m, double *a) 
 }
 
 // ---------------------------------------------------------------------
-// step_kernel: one workgroup per slot.
-// LDS: solve matrix [mfit*mfit], da, prow, dumv [mfit], reduction [8],
-// area/g [F] (for the final curv of a finished fit), ipiv [mfit] ints.
-__global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W) {
+// Gauss-Jordan elimination with full pivoting (period_search/gauss_errc.c:9-73)
+// by one 256-thread workgroup on the augmented matrix [A | b] (n x (n+1),
+// row-major, b as column n) held twice in LDS: each pivot step reads buffer
+// i&1 and writes buffer (i+1)&1, so a step needs one barrier.  Thread
+// (wave w, lane l) owns columns l, l+64, ... and rows w, w+4, ...  While it
+// eliminates, every thread also scans its new entries for the next pivot
+// (|a| over unpivoted rows and columns; ties go to the last entry in
+// row-major order, fabs(a) >= big); the per-wave winners meet in LDS.
+// Only the solution b is needed, so pivoted columns are not updated (the
+// reference's in-place inverse); every value b depends on is computed with
+// the reference's operations in the reference's order:
+//   a[icol][l] *= pivinv; b[icol] *= pivinv;
+//   a[ll][l] -= a[icol][l]*dum; b[ll] -= b[icol]*dum  (dum = a[ll][icol]).
+// Rows irow and icol are swapped by reading the old buffer through the
+// permutation.  Returns 0, 1 (singular) or 2 (zero pivot), uniform across the
+// workgroup; on 0 the solution is in x (LDS or global, n entries).
+// LDS: 2 * n * (n+1) + 16 doubles from buf.
+__device__ inline void pivot_max(double &bv, int &be, double ov, int oe) {
+  if (ov > bv || (ov == bv && oe > be)) {
+    bv = ov;
+    be = oe;
+  }
+}
+
+__device__ int gauss_solve(double *buf, int n, double *x) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ld = n + 1, nn = n * ld;
+  double *red = buf + 2 * nn;  // [2][kWaves] x (value, index)
+  unsigned long long done[2] = {0ull, 0ull};  // pivoted columns (ipiv != 0) = pivoted rows
+  auto is_done = [&](int k) { return (done[k >> 6] >> (k & 63)) & 1ull; };
+  // pivot search over the initial matrix
+  {
+    double bv = -1.0;
+    int be = -1;
+    for (int j = wave; j < n; j += kWaves)
+      for (int c = lane; c < n; c += 64) {
+        const double v = fabs(buf[j * ld + c]);
+        if (v >= bv) {
+          bv = v;
+          be = j * n + c;
+        }
+      }
+    for (int o = 32; o > 0; o >>= 1) pivot_max(bv, be, __shfl_xor(bv, o, 64), __shfl_xor(be, o, 64));
+    if (lane == 0) {
+      red[2 * wave] = bv;
+      red[2 * wave + 1] = (double)be;
+    }
+  }
+  __syncthreads();
+  int prev_irow = -1, prev_icol = -1;
+  for (int i = 0; i < n; i++) {
+    const double *Ao = buf + (i & 1) * nn;
+    double *An = buf + ((i + 1) & 1) * nn;
+    const double *rd = red + (i & 1) * 2 * kWaves;
+    double bv = rd[0];
+    int be = (int)rd[1];
+    for (int w = 1; w < kWaves; w++) pivot_max(bv, be, rd[2 * w], (int)rd[2 * w + 1]);
+    int irow, icol;
+    if (be < 0) {  // no candidate (NaN matrix): the reference keeps the previous pivot
+      if (prev_icol < 0) return 1;
+      irow = prev_irow;
+      icol = prev_icol;
+    } else {
+      irow = be / n;
+      icol = be - irow * n;
+    }
+    if (is_done(icol)) return 1;  // column pivoted twice: the next scan reports singular
+    prev_irow = irow;
+    prev_icol = icol;
+    const double piv = Ao[irow * ld + icol];  // a[icol][icol] after the swap
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    done[icol >> 6] |= 1ull << (icol & 63);
+    double nbv = -1.0;
+    int nbe = -1;
+    for (int c = lane; c <= n; c += 64) {
+      if (c < n && is_done(c)) continue;  // pivoted (incl. icol): never read again
+      const double pr = Ao[irow * ld + c] * pivinv;  // new a[icol][c] (b for c == n)
+      for (int j = wave; j < n; j += kWaves) {
+        double v;
+        if (j == icol) {
+          v = pr;
+        } else {
+          const int sj = (j == irow) ? icol : j;  // row j after the swap
+          v = Ao[sj * ld + c] - pr * Ao[sj * ld + icol];
+        }
+        An[j * ld + c] = v;
+        if (c < n && !is_done(j)) {
+          const double av = fabs(v);
+          if (av >= nbv) {
+            nbv = av;
+            nbe = j * n + c;
+          }
+        }
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) pivot_max(nbv, nbe, __shfl_xor(nbv, o, 64), __shfl_xor(nbe, o, 64));
+    if (lane == 0) {
+      double *wr = red + ((i + 1) & 1) * 2 * kWaves;
+      wr[2 * wave] = nbv;
+      wr[2 * wave + 1] = (double)nbe;
+    }
+    __syncthreads();
+  }
+  const double *Af = buf + (n & 1) * nn;
+  for (int j = tid; j < n; j += kThreads) x[j] = Af[j * ld + n];
+  return 0;
+}
+
+// step_kernel: one workgroup per slot.  Finishes a fit whose loop predicate
+// is false (per-fit summary) and takes the next fit from the queue, or damps
+// the accepted normal equations and solves them, forming the trial
+// parameters (their curv() runs in curv_kernel).
+// LDS: the solver's two augmented matrices and pivot exchange; the final
+// curv of a finished fit (area/g [F]) reuses the same space.
+constexpr int kStepThreads = kThreads;
+__global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWork W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
   const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
   SlotState &S = W.st[slot];
   const SlotMem M = slot_mem(ma, mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
-  double *A = sm, *da = A + mfit * mfit, *prow = da + kMaxMfit, *dumv = prow + kMaxMfit, *red = dumv + kMaxMfit;
-  double *area = red + 2 * kWaves, *g = area + P.nfac;
-  int *ipiv = reinterpret_cast<int *>(g + P.nfac);
+  double *A = sm, *area = sm, *g = area + P.nfac;
   __shared__ int s_phase, s_go, s_item;
 
   if (tid == 0) {
@@ -946,13 +952,11 @@ __global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W)
     }
     double *a = mem + M.a, *atry = mem + M.atry;
     if (W.mode == 1) {
-      for (int l = tid; l < ma; l += kThreads) a[l] = W.cg_in[(size_t)item * ma + l];
+      for (int l = tid; l < ma; l += kStepThreads) a[l] = W.cg_in[(size_t)item * ma + l];
     } else {
       init_params(P, W.freq[item], W.pole[item], a);
     }
-    for (int l = tid; l < ma; l += kThreads) atry[l] = a[l];
-    __syncthreads();
-    eval_curv(P, atry, mem + M.area, mem + M.g);
+    for (int l = tid; l < ma; l += kStepThreads) atry[l] = a[l];
     if (tid == 0) {
       S.item = item;
       S.phase = PH_INIT;
@@ -964,34 +968,50 @@ __global__ void __launch_bounds__(kThreads) step_kernel(DevProblem P, DevWork W)
   // damped normal equations and solve (mrqmin.c:53-79)
   const double *aacc = mem + (S.sel ? M.alpha1 : M.alpha0);
   const double *bacc = mem + (S.sel ? M.beta1 : M.beta0);
+  double *da = mem + M.da;
   {
 #pragma clang fp contract(off)
     const double lam = S.lam;
-    for (int e = tid; e < mfit * mfit; e += kThreads) {
-      const int j = e / mfit, k = e - j * mfit;
-      A[e] = (j == k) ? aacc[e] * (1 + lam) : aacc[e];
+    const int ld = mfit + 1;
+    for (int j = tid >> 6; j < mfit; j += kWaves) {
+      for (int k = tid & 63; k <= mfit; k += 64)
+        A[j * ld + k] = (k == mfit) ? bacc[j] : ((j == k) ? aacc[j * mfit + k] * (1 + lam) : aacc[j * mfit + k]);
     }
-    for (int j = tid; j < mfit; j += kThreads) da[j] = bacc[j];
   }
   __syncthreads();
-  const int err = gauss_solve(A, mfit, da, ipiv, prow, dumv, red);
+  const int err = gauss_solve(A, mfit, da);
   __syncthreads();
   double *a = mem + M.a, *atry = mem + M.atry;
   if (err == 0) {
 #pragma clang fp contract(off)
-    for (int l = tid; l < ma; l += kThreads) atry[l] = a[l];
+    for (int l = tid; l < ma; l += kStepThreads) atry[l] = a[l];
     __syncthreads();
-    for (int jf = tid; jf < mfit; jf += kThreads) {
+    for (int jf = tid; jf < mfit; jf += kStepThreads) {
       const int l = P.fidx[jf];
       atry[l] = a[l] + da[jf];
     }
-    __syncthreads();
-    eval_curv(P, atry, mem + M.area, mem + M.g);
   }
   if (tid == 0) {
     S.eval = err == 0 ? 1 : 0;
     S.err = err;
   }
+}
+
+// curv_kernel: curv() at the trial parameters of every evaluating slot,
+// one thread per facet (period_search/curv.c:10-38, reference operation
+// order); Area and g go to the slot for the model and normal-equation
+// kernels.  blockIdx.x = chunk * nslots + slot.
+__global__ void __launch_bounds__(kThreads) curv_kernel(DevProblem P, DevWork W) {
+#pragma clang fp contract(off)
+  const int slot = blockIdx.x % W.nslots, chunk = blockIdx.x / W.nslots;
+  if (!W.st[slot].eval) return;
+  const int f = chunk * kThreads + threadIdx.x;
+  if (f >= P.nfac) return;
+  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  const double gf = curv_g(P, mem + M.atry, f);
+  mem[M.g + f] = gf;
+  mem[M.area + f] = P.darea[f] * gf;
 }
 
 // model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
@@ -1394,7 +1414,8 @@ size_t mfma_lds(int nfac, int ncoef) {
 }
 
 size_t step_lds(int mfit, int nfac) {
-  return ((size_t)mfit * mfit + 3 * kMaxMfit + 2 * kWaves + 2 * (size_t)nfac + kMaxMfit / 2 + 2) * sizeof(double);
+  const size_t solve = 2 * (size_t)mfit * (mfit + 1) + 4 * kWaves, fin = 2 * (size_t)nfac;
+  return (solve > fin ? solve : fin) * sizeof(double);
 }
 size_t model_lds(int nfac) {
   return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
@@ -1519,7 +1540,10 @@ int ensure_slots(ps_engine *e, int nslots) {
 // Run W.n_items items through the slots until all are done.
 int run_batch(ps_engine *e, DevWork W) {
   if (W.n_items <= 0) return PS_OK;
-  int nslots = e->max_slots < W.n_items ? e->max_slots : W.n_items;
+  // as few refill rounds as max_slots allows, with the items spread evenly
+  // over them (fits of equal length then finish together, no thin tail)
+  const int nrounds = (W.n_items + e->max_slots - 1) / e->max_slots;
+  const int nslots = (W.n_items + nrounds - 1) / nrounds;
   int rc;
   if ((rc = ensure_slots(e, nslots))) return rc;
   W.nslots = nslots;
@@ -1544,7 +1568,9 @@ int run_batch(ps_engine *e, DevWork W) {
   long long it = 0;
   while (true) {
     for (int k = 0; k < poll; k++) {
-      hipLaunchKernelGGL(step_kernel, dim3(nslots), dim3(kThreads), e->lds_step, e->stream, P, W);
+      hipLaunchKernelGGL(step_kernel, dim3(nslots), dim3(kStepThreads), e->lds_step, e->stream, P, W);
+      hipLaunchKernelGGL(curv_kernel, dim3(nslots * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                         e->stream, P, W);
       if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k], e->stream));
       hipLaunchKernelGGL(e->model, dim3(nslots * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads),
                          e->lds_model, e->stream, P, W);
@@ -1702,6 +1728,10 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   for (int l = 0; l < ma; l++)
     if (ia[l]) fidx.push_back(l);
   const int mfit = (int)fidx.size();
+  if (mfit < 1 || mfit > 128) {
+    e->err = "the number of free parameters must be 1..128";
+    return PS_ERR_LIMIT;
+  }
   const int ncp = (ncoef + 7) / 8 * 8;
   const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs)
   e->use_mfma = e->precision == PS_PREC_FP64 && !(kv && !strcmp(kv, "valu"));
