@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector = matrix), dense
-FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector = matrix), dense
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix, dense
+BF16_PEAK_TFLOPS = 2500.0  # MI355X bf16 matrix, dense
 WORKLOAD = "wu"
 
 
@@ -70,7 +71,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
-    ap.add_argument("--prec", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3"), default="fp64")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -88,14 +89,15 @@ def main() -> int:
               f"(the period index shards across GPUs by contiguous ranges, see DESIGN.md)", file=sys.stderr)
         return 2
 
-    from periodsearch_amd import PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16X3, PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
 
     tmp = tempfile.mkdtemp(prefix="psbench_")
     path = os.path.join(tmp, "period_search_in")
     synth.write_workload(WORKLOAD, path)
     inp = Input(path)
     eng = Engine(0)
-    eng.set_precision(PS_PREC_FP32 if args.prec == "fp32" else PS_PREC_FP64)
+    eng.set_precision({"fp64": PS_PREC_FP64, "fp32": PS_PREC_FP32, "bf16": PS_PREC_BF16,
+                       "bf16x3": PS_PREC_BF16X3}[args.prec])
     if args.slots:
         eng.set_slots(args.slots)
     eng.set_problem(inp.problem, keep=inp)
@@ -125,7 +127,8 @@ def main() -> int:
     # model evaluation (masked facets counted, as the reference's dense loop)
     flop_eval = 2.0 * N * F * nc
     achieved = (st["evals"] * flop_eval) / (st["model_ms"] * 1e-3) / 1e12 if st["model_ms"] > 0 else 0.0
-    peak = FP64_PEAK_TFLOPS if args.prec == "fp64" else FP32_PEAK_TFLOPS
+    peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS,
+            "bf16x3": BF16_PEAK_TFLOPS}[args.prec]
     iters = 50
     flop_fit = (iters + 1) * (2.0 * N * F * nc + (N + 3) * mfit * (mfit + 1))
     out = {
@@ -139,12 +142,16 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.prec == "fp64" else "f32 model / f64 solve",
+        "dtype": {"fp64": "f64", "fp32": "f32 contraction and alpha (MFMA, f32 accumulate), f64 geometry/chi2/solve",
+                  "bf16": "bf16 contraction (MFMA, f32 accumulate), f32 facet stage and alpha, f64 geometry/chi2/solve",
+                  "bf16x3": "split-bf16 contraction (3 MFMA products, f32 accumulate), f32 facet stage and alpha, "
+                            "f64 geometry/chi2/solve"}[args.prec],
         "data": "synthetic",
         "config": {"workload": "1080 points (6x80 dense + 600 sparse, 15 yr, 5% noise), lmax=mmax=6, "
                                "nrows=8 (512 facets), 2-100 h, stop 50, 10 poles",
                    "periods_per_step": P, "fits_per_step": P * 10, "precision": args.prec,
-                   "form": "plain FP64" if args.prec == "fp64" else "FP32 model stage, FP64 solve and chi2"},
+                   "form": {"fp64": "plain FP64", "fp32": "mixed FP32", "bf16": "mixed BF16",
+                            "bf16x3": "mixed split BF16"}[args.prec]},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": "model_kernel", "model_ms_total": round(st["model_ms"], 3),

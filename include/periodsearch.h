@@ -44,11 +44,18 @@ extern "C" {
 #define PS_ERR_STATE -5
 
 /* Arithmetic of the model/Jacobian stage.  PS_PREC_FP64 is the plain form
- * (everything FP64); PS_PREC_FP32 runs the facet stage and the
- * brightness/derivative contraction in FP32 (phase reduction, normal
- * equations, solve, chi2 and acceptance stay FP64). */
+ * (everything FP64).  The mixed forms keep the per-point geometry (phase
+ * reduction, rotation, phase function), the model values, residuals, chi2,
+ * beta, the normal-equation solve and the LM acceptance in FP64; the facet
+ * stage runs in FP32, the brightness/derivative contraction and the alpha
+ * accumulation on the matrix cores with FP32 accumulation, with FP32
+ * operands (PS_PREC_FP32), bf16 operands for the contraction
+ * (PS_PREC_BF16; alpha stays FP32) or the contraction operands split into
+ * bf16 high and low parts, three bf16 products (PS_PREC_BF16X3). */
 #define PS_PREC_FP64 0
 #define PS_PREC_FP32 1
+#define PS_PREC_BF16 2
+#define PS_PREC_BF16X3 3
 
 typedef struct ps_engine ps_engine;
 

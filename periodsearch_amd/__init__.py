@@ -29,6 +29,8 @@ DRIVER_PATH = os.path.join(HERE, "period_search_amd")
 
 PS_PREC_FP64 = 0
 PS_PREC_FP32 = 1
+PS_PREC_BF16 = 2
+PS_PREC_BF16X3 = 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

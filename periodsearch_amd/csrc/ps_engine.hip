@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "periodsearch.h"
@@ -53,6 +54,8 @@ constexpr double kTiny = 1e-8;     // period_search/constants.h:10
 constexpr double kPi = 3.14159265358979323846;
 
 typedef double v4d __attribute__((ext_vector_type(4)));  // one 16x16 FP64 MFMA accumulator per lane
+typedef float v4f __attribute__((ext_vector_type(4)));   // one 16x16 FP32-accumulate MFMA accumulator per lane
+typedef __bf16 v8bf __attribute__((ext_vector_type(8))); // bf16 operand of v_mfma_f32_16x16x32_bf16
 
 #define HIP_TRY(x)                                                         \
   do {                                                                     \
@@ -73,7 +76,12 @@ struct DevProblem {
   // facets
   const double *nx, *ny, *nz, *darea;
   const double *dsph;     // [F][ncp], zero-padded columns
-  const float *nxf, *nyf, *nzf, *dsphf;  // FP32 copies for the reduced-precision model
+  // mixed-precision tables: normals, and Dsph transposed [16*nb][fpad] (zero
+  // beyond ncoef and beyond F) in FP32 and in bf16
+  const float4 *nor4f;
+  const float *dsphT32;
+  const unsigned short *dsphT16, *dsphT16lo;  // bf16 of Dsph, and of Dsph - hi (split form)
+  int fpad;
   const double4 *nor4;    // [F] (nx, ny, nz, 0): one 32-byte scalar load per facet
   const double *cpleg;    // [ngrp][F]: P_l^m of coefficient group q at facet f
   const double *cfc;      // [mmax+1][F]
@@ -362,221 +370,71 @@ __device__ void model_tile(const DevProblem &P, int p0, const double *a, const d
   J[(size_t)P.ma * ld + p] = br * sc;
 }
 
-// Reduced-precision pass 1 (PS_PREC_FP32): the phase reduction, rotation
-// matrices and phase function stay FP64 (point_geometry); the facet loop and
-// the brightness-derivative contraction run in FP32; rows are stored as FP64
-// so the normal equations, solve, chi2 and the LM decisions stay FP64.
-template <int NCB>
-__device__ void model_tile_f32(const DevProblem &P, int p0, const double *a, const double *blm, const double *area,
-                               double *J, PointGeom *gst) {
-  const int lane_p = p0 + (int)threadIdx.x;
-  const int wave_first = p0 + ((int)threadIdx.x & ~63);
-  if (wave_first >= P.n_points) return;
-  const int p = lane_p < P.n_points ? lane_p : P.n_points - 1;
-  {
-    PointGeom G;
-    point_geometry(P, p, a, blm, G);
-    gst[threadIdx.x] = G;  // parked in LDS while the facet loop runs
-  }
-  const float cl = (float)exp(a[P.ma - 2]);
-  const float cls = (float)a[P.ma - 1];
-  const float tiny = (float)kTiny;
-  float S[NCB];
-#pragma unroll
-  for (int c = 0; c < NCB; c++) S[c] = 0.f;
-  float br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
-  const float e_0 = (float)gst[threadIdx.x].e[0], e_1 = (float)gst[threadIdx.x].e[1],
-              e_2 = (float)gst[threadIdx.x].e[2];
-  const float s_0 = (float)gst[threadIdx.x].e0[0], s_1 = (float)gst[threadIdx.x].e0[1],
-              s_2 = (float)gst[threadIdx.x].e0[2];
-  const float *__restrict__ nxp = P.nxf;
-  const float *__restrict__ nyp = P.nyf;
-  const float *__restrict__ nzp = P.nzf;
-  const float *__restrict__ dsp = P.dsphf;
-#pragma unroll 1
-  for (int f = 0; f < P.nfac; f++) {
-    const float n0 = nxp[f], n1 = nyp[f], n2 = nzp[f];
-    const float mu = e_0 * n0 + e_1 * n1 + e_2 * n2;
-    const float mu0 = s_0 * n0 + s_1 * n1 + s_2 * n2;
-    if (mu > tiny && mu0 > tiny) {
-      const float af = (float)area[f];
-      const float rd = 1.0f / (mu + mu0);
-      const float mm = mu * mu0;
-      const float s = mm * (cl + cls * rd);
-      const float wgt = af * s;
-      br += wgt;
-      const float q0 = mu0 * rd, q1 = mu * rd;
-      const float ad = af * (cls * q0 * q0 + cl * mu0);
-      const float ad0 = af * (cls * q1 * q1 + cl * mu);
-      v0 += ad * n0;
-      v1 += ad * n1;
-      v2 += ad * n2;
-      u0 += ad0 * n0;
-      u1 += ad0 * n1;
-      u2 += ad0 * n2;
-      const float am = af * mm;
-      scl += am;
-      scls += am * rd;
-      const float *drow = dsp + (size_t)f * NCB;
-#pragma unroll
-      for (int c = 0; c < NCB; c++) S[c] += wgt * drow[c];
-    }
-  }
-  if (lane_p >= P.n_points) return;
-  const PointGeom G = gst[threadIdx.x];
-  const int ld = P.npad;
-  const double sc = G.scale;
-#pragma unroll
-  for (int c = 0; c < NCB; c++)
-    if (c < P.ncoef) J[(size_t)c * ld + p] = sc * (double)S[c];
-  const double V[3] = {v0, v1, v2}, U[3] = {u0, u1, u2};
-  double r[3];
-  rotation_derivs(P, p, blm, G, V, U, r);
-  for (int k = 0; k < 3; k++) J[(size_t)(P.ncoef + k) * ld + p] = sc * r[k];
-  const double brd = br;
-  for (int i = 0; i < 3; i++) J[(size_t)(P.ncoef + 3 + i) * ld + p] = brd * G.dphp[i];
-  J[(size_t)(P.ncoef + 6) * ld + p] = sc * (double)scl * (double)cl;
-  J[(size_t)(P.ncoef + 7) * ld + p] = sc * (double)scls;
-  J[(size_t)P.ma * ld + p] = brd * sc;
-}
-
 // The three convexity rows (period_search/conv.c:12-29) at points
 // n_points..n_points+2, and chck = sum Area*Nor.  Serial over facets in the
-// reference order.
+// reference order.  MIX: derivative rows stored as FP32 (the model value row
+// ma stays FP64).
+template <bool MIX>
 __device__ void conv_rows(const DevProblem &P, const double *area, const double *g, double *J, double *chck) {
 #pragma clang fp contract(off)
   const int tid = threadIdx.x, ld = P.npad, n = P.n_points;
+  float *Jf = reinterpret_cast<float *>(J);
+  auto put = [&](int c, int q, double v) {
+    if (MIX)
+      Jf[(size_t)c * ld + q] = (float)v;
+    else
+      J[(size_t)c * ld + q] = v;
+  };
+  // facets in the reference order; loads issued eight facets at a time
+  const int F = P.nfac;
   if (tid < P.ncoef) {
     double r0 = 0, r1 = 0, r2 = 0;
-    for (int f = 0; f < P.nfac; f++) {
-      double dg = g[f] * P.dsph[(size_t)f * P.ncp + tid];
-      double t = P.darea[f] * dg;
-      r0 += t * P.nx[f];
-      r1 += t * P.ny[f];
-      r2 += t * P.nz[f];
+    for (int f0 = 0; f0 < F; f0 += 8) {
+      double gv[8], dv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int f = min(f0 + u, F - 1);
+        gv[u] = g[f];
+        dv[u] = P.dsph[(size_t)f * P.ncp + tid];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int f = f0 + u;
+        if (f < F) {
+          double dg = gv[u] * dv[u];
+          double t = P.darea[f] * dg;
+          r0 += t * P.nx[f];
+          r1 += t * P.ny[f];
+          r2 += t * P.nz[f];
+        }
+      }
     }
-    J[(size_t)tid * ld + n] = r0;
-    J[(size_t)tid * ld + n + 1] = r1;
-    J[(size_t)tid * ld + n + 2] = r2;
+    put(tid, n, r0);
+    put(tid, n + 1, r1);
+    put(tid, n + 2, r2);
   } else if (tid < P.ma) {
-    J[(size_t)tid * ld + n] = 0;
-    J[(size_t)tid * ld + n + 1] = 0;
-    J[(size_t)tid * ld + n + 2] = 0;
+    put(tid, n, 0);
+    put(tid, n + 1, 0);
+    put(tid, n + 2, 0);
   } else if (tid >= kThreads - 3) {
     const int nc = tid - (kThreads - 3);
     const double *nv = nc == 0 ? P.nx : (nc == 1 ? P.ny : P.nz);
     double r = 0;
-    for (int f = 0; f < P.nfac; f++) r += area[f] * nv[f];
+    for (int f0 = 0; f0 < F; f0 += 8) {
+      double av[8], nn[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int f = min(f0 + u, F - 1);
+        av[u] = area[f];
+        nn[u] = nv[f];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (f0 + u < F) r += av[u] * nn[u];
+    }
     J[(size_t)P.ma * ld + n + nc] = r;
     chck[nc] = r;
   }
-}
-
-// Lower-triangle entry e -> (j, k), j >= k, e = j(j+1)/2 + k.
-__device__ inline void tri_index(int e, int &j, int &k) {
-  int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-  while ((r + 1) * (r + 2) / 2 <= e) r++;
-  while (r * (r + 1) / 2 > e) r--;
-  j = r;
-  k = e - r * (r + 1) / 2;
-}
-
-// Pass 2: relative renormalisation and the normal equations
-// (period_search/mrqcof.c:42-139).  Points are staged kChunk at a time in
-// LDS as renormalised rows Jn and weighted rows WT = Jn * sig2i * w; each
-// lower-triangle entry of alpha is owned by one thread and summed serially
-// over points in the reference's order, accumulating in the per-slot alpha
-// buffer (global, L2-resident).  beta and chi2 likewise.  alpha is
-// mirrored to full symmetric storage at the end.
-__device__ void normal_equations(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
-                                 double *beta, double *chi_out, double *alpha) {
-#pragma clang fp contract(off)
-  const int tid = threadIdx.x, ld = P.npad, mfit = P.mfit, ma = P.ma;
-  const int nent = mfit * (mfit + 1) / 2;
-  double bacc = 0, chi = 0;
-  bool first = true;
-  double *Jn = wst;                          // [kChunk][mfit]
-  double *WT = wst + kChunk * mfit;          // [kChunk][mfit]
-  double *aux = wst + 2 * kChunk * mfit;     // [kChunk][3]: sig2i, weight, dy
-  for (int lc = 0; lc <= P.n_lc; lc++) {
-    const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
-    const int inrel = P.lc_inrel[lc];
-    if (inrel) {
-      // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda
-      if (tid <= ma) {
-        const double *col = J + (size_t)tid * ld;
-        double s = 0;
-        for (int q = q0; q < q1; q++) s = s + col[q];
-        dave[tid] = s;
-      }
-      __syncthreads();
-    }
-    for (int c0 = q0; c0 < q1; c0 += kChunk) {
-      const int cn = min(kChunk, q1 - c0);
-      for (int pp = tid; pp < cn; pp += kThreads) {
-        const int q = c0 + pp;
-        double ymod = J[(size_t)ma * ld + q];
-        double sg, wg, yo;
-        if (q < P.n_points) {
-          sg = P.sig[q];
-          wg = P.w[q];
-          yo = P.y[q];
-        } else {
-          sg = 1 / conw_r;
-          wg = 1;
-          yo = 0;
-        }
-        if (inrel) {
-          const double ave = dave[ma];
-          const double coef = sg * np / ave;
-          ymod = coef * ymod;
-        }
-        aux[pp * 3 + 0] = 1 / (sg * sg);
-        aux[pp * 3 + 1] = wg;
-        aux[pp * 3 + 2] = yo - ymod;
-      }
-      __syncthreads();
-      for (int idx = tid; idx < cn * mfit; idx += kThreads) {
-        const int pp = idx / mfit, jf = idx - pp * mfit;
-        const int q = c0 + pp;
-        const int l = P.fidx[jf];
-        double d = J[(size_t)l * ld + q];
-        if (inrel) {
-          const double ave = dave[ma];
-          const double coef = (q < P.n_points ? P.sig[q] : 1 / conw_r) * np / ave;
-          d = (l == 0) ? 0.0 : coef * (d - J[(size_t)ma * ld + q] * dave[l] / ave);
-        }
-        Jn[pp * mfit + jf] = d;
-        WT[pp * mfit + jf] = d * aux[pp * 3] * aux[pp * 3 + 1];
-      }
-      __syncthreads();
-      for (int e = tid; e < nent; e += kThreads) {
-        int j, k;
-        tri_index(e, j, k);
-        double s = first ? 0.0 : alpha[j * mfit + k];
-        for (int pp = 0; pp < cn; pp++) s = s + WT[pp * mfit + j] * Jn[pp * mfit + k];
-        alpha[j * mfit + k] = s;
-      }
-      if (tid < mfit) {
-        for (int pp = 0; pp < cn; pp++) bacc = bacc + aux[pp * 3 + 2] * WT[pp * mfit + tid];
-      }
-      if (tid == kThreads - 1) {
-        for (int pp = 0; pp < cn; pp++) {
-          const double dy = aux[pp * 3 + 2];
-          chi = chi + dy * dy * aux[pp * 3] * aux[pp * 3 + 1];
-        }
-      }
-      first = false;
-      __syncthreads();
-    }
-  }
-  for (int e = tid; e < nent; e += kThreads) {
-    int j, k;
-    tri_index(e, j, k);
-    if (j != k) alpha[k * mfit + j] = alpha[j * mfit + k];
-  }
-  if (tid < mfit) beta[tid] = bacc;
-  if (tid == kThreads - 1) *chi_out = chi;
 }
 
 // Pass 2 on the matrix cores: relative renormalisation (mrqcof.c:42-104)
@@ -589,28 +447,46 @@ __device__ void normal_equations(const DevProblem &P, const double *J, double co
 // (tile t on wave t%4, at most NT per wave); alpha takes the j >= k entries
 // and mirrors them, as the reference does.  beta and chi2 are summed by
 // single threads in point order.
+// MIX (mixed precision): the derivative rows are FP32, Jn is staged as FP32
+// and alpha is summed by v_mfma_f32_16x16x4f32 (FP32 accumulation); the
+// light-curve sums, beta, chi2 and the residuals stay FP64.
 constexpr int kNeqChunk = 64;
 
-template <int NT>
+// Doubles taken by the staged rows Jn (also the light-curve-sum scratch).
+__host__ __device__ inline int neq_jn_doubles(int mfit, bool mix) {
+  const int MS = (mfit + 15) / 16 * 16 + 1;
+  int d = mix ? (kNeqChunk * MS + 1) / 2 : kNeqChunk * MS;
+  return d < 4 * (kMaxMa + 1) ? 4 * (kMaxMa + 1) : d;
+}
+
+template <int NT, bool MIX>
 __device__ void normal_equations_mfma(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
-                                      double *beta, double *chi_out, double *alpha) {
+                                      double *drat, double *beta, double *chi_out, double *alpha) {
 #pragma clang fp contract(off)
+  using AccT = typename std::conditional<MIX, v4f, v4d>::type;
+  using JnT = typename std::conditional<MIX, float, double>::type;
+  const float *Jf = reinterpret_cast<const float *>(J);
+  // entry (l, q) of the Jacobian, l <= ma (row ma = ymod is FP64 in both forms)
+  auto jv = [&](int l, int q) -> double {
+    if (MIX && l < P.ma) return (double)Jf[(size_t)l * P.npad + q];
+    return J[(size_t)l * P.npad + q];
+  };
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ld = P.npad, mfit = P.mfit, ma = P.ma;
   const int nb = (mfit + 15) / 16, MP = 16 * nb, MS = MP + 1;
   const int ntile = nb * (nb + 1) / 2;
-  v4d acc[NT];
+  AccT acc[NT];
   int tj[NT], tk[NT];
 #pragma unroll
   for (int i = 0; i < NT; i++) {
-    acc[i] = v4d{0, 0, 0, 0};
+    acc[i] = AccT{0, 0, 0, 0};
     int t = wave + 4 * i, jb = 0;
     while ((jb + 1) * (jb + 2) / 2 <= t) jb++;
     tj[i] = jb;
     tk[i] = t - jb * (jb + 1) / 2;
   }
-  double *Jn = wst;                        // [kNeqChunk][MS]
-  double *aux = wst + kNeqChunk * MS;      // [kNeqChunk][3]: sig2i, weight, dy
+  JnT *Jn = reinterpret_cast<JnT *>(wst);       // [kNeqChunk][MS]
+  double *aux = wst + neq_jn_doubles(mfit, MIX);  // [kNeqChunk][3]: sig2i, weight, dy
   double bacc = 0, chi = 0;
   for (int lc = 0; lc <= P.n_lc; lc++) {
     const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
@@ -619,17 +495,26 @@ __device__ void normal_equations_mfma(const DevProblem &P, const double *J, doub
       // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda.
       // Each column is summed by 4 threads over interleaved quarters of the
       // points (8 independent loads in flight each), then combined in order.
-      double *part = Jn;  // [4][ma+1] scratch, free before the chunk loop
+      double *part = wst;  // [4][ma+1] scratch, free before the chunk loop
       for (int cs = tid; cs < 4 * (ma + 1); cs += kThreads) {
         const int col = cs % (ma + 1), seg = cs / (ma + 1);
-        const double *cp = J + (size_t)col * ld;
         double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         int q = q0 + seg;
-        for (; q + 28 < q1; q += 32) {
+        if (MIX && col < ma) {
+          const float *cp = Jf + (size_t)col * ld;
+          for (; q + 28 < q1; q += 32) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) s8[u] = s8[u] + cp[q + 4 * u];
+            for (int u = 0; u < 8; u++) s8[u] = s8[u] + (double)cp[q + 4 * u];
+          }
+          for (int u = 0; q < q1; q += 4, u++) s8[u & 7] = s8[u & 7] + (double)cp[q];
+        } else {
+          const double *cp = J + (size_t)col * ld;
+          for (; q + 28 < q1; q += 32) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) s8[u] = s8[u] + cp[q + 4 * u];
+          }
+          for (int u = 0; q < q1; q += 4, u++) s8[u & 7] = s8[u & 7] + cp[q];
         }
-        for (int u = 0; q < q1; q += 4, u++) s8[u & 7] = s8[u & 7] + cp[q];
         double sum = 0;
 #pragma unroll
         for (int u = 0; u < 8; u++) sum = sum + s8[u];
@@ -639,16 +524,25 @@ __device__ void normal_equations_mfma(const DevProblem &P, const double *J, doub
       for (int col = tid; col <= ma; col += kThreads)
         dave[col] = ((part[col] + part[(ma + 1) + col]) + part[2 * (ma + 1) + col]) + part[3 * (ma + 1) + col];
       __syncthreads();
+      if (MIX) {  // dave[l]/ave, once per light curve
+        for (int col = tid; col < ma; col += kThreads) drat[col] = dave[col] / dave[ma];
+        __syncthreads();
+      }
     }
     for (int c0 = q0; c0 < q1; c0 += kNeqChunk) {
       const int cn = min(kNeqChunk, q1 - c0);
       const int cn4 = (cn + 3) & ~3;
-      for (int pp = tid; pp < cn4; pp += kThreads) {
-        double s2 = 0, wg = 0, dy = 0;
-        if (pp < cn) {
-          const int q = c0 + pp;
-          double ymod = J[(size_t)ma * ld + q];
-          double sg, yo;
+      // lane = point of the chunk (kNeqChunk = 64): per-point constants,
+      // then the renormalised rows, one wave-uniform row jf at a time
+      // (coalesced over points).  mrqcof.c:92-104 order:
+      // coef = sig*np/ave; dyda = coef*(dyda - ymod*dave[l]/ave); ymod *= coef.
+      {
+        const int pp = lane, q = c0 + pp;
+        const bool live = pp < cn;
+        double ymr = 0, coef = 0;
+        if (live) {
+          ymr = J[(size_t)ma * ld + q];
+          double sg, wg, yo;
           if (q < P.n_points) {
             sg = P.sig[q];
             wg = P.w[q];
@@ -658,50 +552,61 @@ __device__ void normal_equations_mfma(const DevProblem &P, const double *J, doub
             wg = 1;
             yo = 0;
           }
-          if (inrel) {
-            const double ave = dave[ma];
-            const double coef = sg * np / ave;
-            ymod = coef * ymod;
+          if (inrel) coef = sg * np / dave[ma];
+          if (wave == 0) {
+            aux[pp * 3 + 0] = 1 / (sg * sg);
+            aux[pp * 3 + 1] = wg;
+            aux[pp * 3 + 2] = yo - (inrel ? coef * ymr : ymr);
           }
-          s2 = 1 / (sg * sg);
-          dy = yo - ymod;
+        } else if (wave == 0 && pp < cn4) {
+          aux[pp * 3 + 0] = 0;
+          aux[pp * 3 + 1] = 0;
+          aux[pp * 3 + 2] = 0;
         }
-        aux[pp * 3 + 0] = s2;
-        aux[pp * 3 + 1] = wg;
-        aux[pp * 3 + 2] = dy;
-      }
-      // renormalised rows, coalesced over points
-      for (int idx = tid; idx < MP * cn4; idx += kThreads) {
-        const int jf = idx / cn4, pp = idx - jf * cn4;
-        double d = 0.0;
-        if (jf < mfit && pp < cn) {
-          const int q = c0 + pp;
-          const int l = P.fidx[jf];
-          d = J[(size_t)l * ld + q];
-          if (inrel) {
-            const double ave = dave[ma];
-            const double coef = (q < P.n_points ? P.sig[q] : 1 / conw_r) * np / ave;
-            d = (l == 0) ? 0.0 : coef * (d - J[(size_t)ma * ld + q] * dave[l] / ave);
+        const double ave = dave[ma];
+        // eight rows per batch: their loads are issued together
+        for (int jf0 = wave; jf0 < MP; jf0 += 8 * kWaves) {
+          double dv[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int jf = jf0 + kWaves * u;
+            dv[u] = (jf < mfit && live) ? jv(P.fidx[jf], q) : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int jf = jf0 + kWaves * u;
+            if (jf >= MP) break;
+            double d = dv[u];
+            if (inrel && jf < mfit && live) {
+              const int l = P.fidx[jf];
+              if (MIX)
+                d = (l == 0) ? 0.0 : coef * (d - ymr * drat[l]);
+              else
+                d = (l == 0) ? 0.0 : coef * (d - ymr * dave[l] / ave);
+            }
+            if (pp < cn4) Jn[pp * MS + jf] = (JnT)d;
           }
         }
-        Jn[pp * MS + jf] = d;
       }
       __syncthreads();
       for (int kk = 0; kk < cn4 / 4; kk++) {
         const int row = 4 * kk + (lane >> 4);
-        const double s2 = aux[row * 3], wg = aux[row * 3 + 1];
+        const JnT s2 = (JnT)aux[row * 3], wg = (JnT)aux[row * 3 + 1];
 #pragma unroll
         for (int i = 0; i < NT; i++) {
           if (wave + 4 * i < ntile) {
-            const double a = Jn[row * MS + 16 * tj[i] + (lane & 15)] * s2 * wg;
-            const double b = Jn[row * MS + 16 * tk[i] + (lane & 15)];
-            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+            const JnT a = Jn[row * MS + 16 * tj[i] + (lane & 15)] * s2 * wg;
+            const JnT b = Jn[row * MS + 16 * tk[i] + (lane & 15)];
+            if constexpr (MIX)
+              acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
+            else
+              acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
           }
         }
       }
       if (tid < mfit) {
         for (int pp = 0; pp < cn; pp++) {
-          const double wt = Jn[pp * MS + tid] * aux[pp * 3] * aux[pp * 3 + 1];
+          const double wt = (double)Jn[pp * MS + tid] * aux[pp * 3] * aux[pp * 3 + 1];
           bacc = bacc + aux[pp * 3 + 2] * wt;
         }
       }
@@ -719,11 +624,12 @@ __device__ void normal_equations_mfma(const DevProblem &P, const double *J, doub
     if (wave + 4 * i < ntile) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int j = 16 * tj[i] + 4 * r + (lane >> 4);
+        // result rows: FP64 16x16x4 D[4r + l/16][l%16]; FP32 D[4(l/16) + r][l%16]
+        const int j = 16 * tj[i] + (MIX ? 4 * (lane >> 4) + r : 4 * r + (lane >> 4));
         const int k = 16 * tk[i] + (lane & 15);
         if (j < mfit && k < mfit && j >= k) {
-          alpha[j * mfit + k] = acc[i][r];
-          alpha[k * mfit + j] = acc[i][r];
+          alpha[j * mfit + k] = (double)acc[i][r];
+          alpha[k * mfit + j] = (double)acc[i][r];
         }
       }
     }
@@ -842,20 +748,29 @@ __device__ int gauss_solve(double *buf, int n, double *x) {
     for (int c = lane; c <= n; c += 64) {
       if (c < n && is_done(c)) continue;  // pivoted (incl. icol): never read again
       const double pr = Ao[irow * ld + c] * pivinv;  // new a[icol][c] (b for c == n)
-      for (int j = wave; j < n; j += kWaves) {
-        double v;
-        if (j == icol) {
-          v = pr;
-        } else {
-          const int sj = (j == irow) ? icol : j;  // row j after the swap
-          v = Ao[sj * ld + c] - pr * Ao[sj * ld + icol];
+      // rows j = wave + 4u, eight at a time with their LDS loads issued together
+      for (int j0 = wave; j0 < n; j0 += 8 * kWaves) {
+        double av_[8], ac_[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int j = min(j0 + kWaves * u, n - 1);
+          const int sj = (j == irow) ? icol : ((j == icol) ? irow : j);  // row j after the swap
+          av_[u] = Ao[sj * ld + c];
+          ac_[u] = Ao[sj * ld + icol];
         }
-        An[j * ld + c] = v;
-        if (c < n && !is_done(j)) {
-          const double av = fabs(v);
-          if (av >= nbv) {
-            nbv = av;
-            nbe = j * n + c;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int j = j0 + kWaves * u;
+          if (j < n) {
+            const double v = (j == icol) ? pr : av_[u] - pr * ac_[u];
+            An[j * ld + c] = v;
+            if (c < n && !is_done(j)) {
+              const double a = fabs(v);
+              if (a >= nbv) {
+                nbv = a;
+                nbe = j * n + c;
+              }
+            }
           }
         }
       }
@@ -1018,7 +933,7 @@ __global__ void __launch_bounds__(kThreads) curv_kernel(DevProblem P, DevWork W)
 #ifndef PS_MODEL_MIN_WAVES
 #define PS_MODEL_MIN_WAVES 1
 #endif
-template <int NCB, bool F32>
+template <int NCB>
 __global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(DevProblem P, DevWork W) {
   extern __shared__ double sm[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
@@ -1032,10 +947,7 @@ __global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(Dev
   if (threadIdx.x == 0) eval_blmatrix(x[P.ncoef], x[P.ncoef + 1], blm);
   __syncthreads();
   (void)g;
-  if constexpr (F32)
-    model_tile_f32<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
-  else
-    model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
+  model_tile<NCB>(P, tile * kThreads, x, blm, area, mem + M.J, gst);
 }
 
 // model_mfma_kernel: one workgroup per (slot, tile of kTilePts points);
@@ -1190,11 +1102,247 @@ __global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P, DevW
   }
 }
 
+// Mixed-precision model (PS_PREC_FP32, PS_PREC_BF16, PS_PREC_BF16X3).
+// Per-point geometry (phase reduction, rotation matrices, phase function)
+// stays FP64 (point_geometry); the facet stage runs in FP32 and the
+// shape-derivative contraction S = W Dsph on the matrix cores with FP32
+// accumulation, with W and Dsph in FP32 (MODE 0: v_mfma_f32_16x16x4f32,
+// 8 k-steps per 32-facet chunk), in bf16 (MODE 1: v_mfma_f32_16x16x32_bf16,
+// one k-step per chunk) or split into bf16 high and low parts (MODE 2:
+// x = hi + lo, S = Whi Dhi + Whi Dlo + Wlo Dhi, three bf16 products per
+// chunk, about 16 significant bits).  Derivative
+// rows are stored as FP32; the model value ymod (row ma) stays FP64, so the
+// residuals, chi2 and the LM decisions are FP64.
+//   facet stage: thread (point pl, sub-slot s) takes facets f0+8s..f0+8s+7
+//     of each chunk and writes their weights as one 16/32-byte row piece;
+//   contraction: wave w owns points 16w..16w+15; A = W rows, B = Dsph^T
+//     rows staged from the pre-converted table.  Operand layout (A lane l:
+//     row l%16, k-group l/16; B lane l: column l%16, k-group l/16) and
+//     result layout D[4*(l/16)+r][l%16].
+// LDS layout in bytes (mixed_layout): x, blm, area (FP32), geometry, W tile
+// [kTilePts][WS], Dsph^T chunk [16 NB][WS]; the tiles are reused for the
+// partial-sum reduction.
+struct MixedLds {
+  size_t x, blm, area, geom, wt, dch, red, total;
+};
+__host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+__host__ __device__ inline MixedLds mixed_layout(int nfac, int nb, int mode) {
+  MixedLds L;
+  size_t o = 0;
+  const bool bf = mode != 0;
+  const size_t es = bf ? 2 : 4, ws = kFacetChunk + (bf ? 8 : 4), parts = mode == 2 ? 2 : 1;
+  L.x = o;
+  o += kMaxMa * sizeof(double);
+  L.blm = o;
+  o += 32 * sizeof(double);
+  L.area = o;
+  o = align16(o + (size_t)nfac * sizeof(float));
+  L.geom = o;
+  o = align16(o + kTilePts * sizeof(PointGeom));
+  L.wt = o;
+  o = align16(o + parts * kTilePts * ws * es);
+  L.dch = o;
+  o = align16(o + parts * 16 * nb * ws * es);
+  L.red = L.wt;
+  const size_t red = (size_t)kWaves * kTilePts * 9 * sizeof(float);
+  if (o - L.wt < red) o = L.wt + red;
+  L.total = o;
+  return L;
+}
+
+__device__ inline unsigned bf16_bits(float v) {  // round to nearest even
+  const unsigned u = __float_as_uint(v);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P, DevWork W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
+  const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
+  if (!W.st[slot].eval) return;
+  constexpr bool BF = MODE != 0, SPLIT = MODE == 2;
+  constexpr int NC = 16 * NB;
+  constexpr int WS = kFacetChunk + (BF ? 8 : 4);   // padded row, elements (80 / 144 bytes)
+  using T = typename std::conditional<BF, unsigned short, float>::type;
+  constexpr int PER16 = 16 / (int)sizeof(T);      // elements per 16-byte piece
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  const int F = P.nfac;
+  const MixedLds L = mixed_layout(F, NB, MODE);
+  double *x = reinterpret_cast<double *>(smb + L.x), *blm = reinterpret_cast<double *>(smb + L.blm);
+  float *areaf = reinterpret_cast<float *>(smb + L.area);
+  PointGeom *gst = reinterpret_cast<PointGeom *>(smb + L.geom);
+  T *wt = reinterpret_cast<T *>(smb + L.wt);     // [kTilePts][WS] (then the low parts)
+  T *dch = reinterpret_cast<T *>(smb + L.dch);   // [NC][WS] (then the low parts)
+  T *wtl = wt + kTilePts * WS, *dchl = dch + NC * WS;
+  float *red = reinterpret_cast<float *>(smb + L.red);
+
+  for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
+  for (int f = tid; f < F; f += kThreads) areaf[f] = (float)mem[M.area + f];
+  if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
+  __syncthreads();
+  const int p0 = tile * kTilePts;
+  if (tid < kTilePts) {
+    const int p = min(p0 + tid, P.n_points - 1);
+    PointGeom G;
+    point_geometry(P, p, x, blm, G);
+    gst[tid] = G;
+  }
+  __syncthreads();
+  const int pl = tid % kTilePts, sub = tid / kTilePts;
+  const float e_0 = (float)gst[pl].e[0], e_1 = (float)gst[pl].e[1], e_2 = (float)gst[pl].e[2];
+  const float s_0 = (float)gst[pl].e0[0], s_1 = (float)gst[pl].e0[1], s_2 = (float)gst[pl].e0[2];
+  const double cld = exp(x[P.ma - 2]);
+  const float cl = (float)cld, cls = (float)x[P.ma - 1], tiny = (float)kTiny;
+  float br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
+  v4f acc[NB];
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) acc[cb] = v4f{0, 0, 0, 0};
+  const float4 *__restrict__ n4 = P.nor4f;
+  const T *__restrict__ dsT = BF ? reinterpret_cast<const T *>(P.dsphT16) : reinterpret_cast<const T *>(P.dsphT32);
+  const T *__restrict__ dsTl = reinterpret_cast<const T *>(P.dsphT16lo);
+  const int fpad = P.fpad;
+  for (int f0 = 0; f0 < F; f0 += kFacetChunk) {
+    // stage the chunk's Dsph^T rows (fpad is a multiple of the chunk)
+    for (int idx = tid; idx < NC * (kFacetChunk / PER16); idx += kThreads) {
+      const int c = idx / (kFacetChunk / PER16), pc = idx - c * (kFacetChunk / PER16);
+      *reinterpret_cast<uint4 *>(dch + c * WS + pc * PER16) =
+          *reinterpret_cast<const uint4 *>(dsT + (size_t)c * fpad + f0 + pc * PER16);
+      if (SPLIT)
+        *reinterpret_cast<uint4 *>(dchl + c * WS + pc * PER16) =
+            *reinterpret_cast<const uint4 *>(dsTl + (size_t)c * fpad + f0 + pc * PER16);
+    }
+    // facet stage (FP32)
+    float wv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int f = f0 + 8 * sub + i;
+      float w = 0.0f;
+      if (f < F) {
+        const float4 nv = n4[f];
+        const float mu = e_0 * nv.x + e_1 * nv.y + e_2 * nv.z;
+        const float mu0 = s_0 * nv.x + s_1 * nv.y + s_2 * nv.z;
+        if (mu > tiny && mu0 > tiny) {
+          const float af = areaf[f];
+          const float rd = __builtin_amdgcn_rcpf(mu + mu0);
+          const float mm = mu * mu0;
+          w = af * (mm * (cl + cls * rd));
+          br += w;
+          const float q0 = mu0 * rd, q1 = mu * rd;
+          const float ad = af * (cls * q0 * q0 + cl * mu0);
+          const float ad0 = af * (cls * q1 * q1 + cl * mu);
+          v0 += ad * nv.x;
+          v1 += ad * nv.y;
+          v2 += ad * nv.z;
+          u0 += ad0 * nv.x;
+          u1 += ad0 * nv.y;
+          u2 += ad0 * nv.z;
+          const float am = af * mm;
+          scl += am;
+          scls += am * rd;
+        }
+      }
+      wv[i] = w;
+    }
+    if constexpr (BF) {
+      unsigned h[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) h[i] = bf16_bits(wv[i]);
+      *reinterpret_cast<uint4 *>(wt + pl * WS + 8 * sub) =
+          make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+      if constexpr (SPLIT) {
+        unsigned lo[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) lo[i] = bf16_bits(wv[i] - __uint_as_float(h[i] << 16));
+        *reinterpret_cast<uint4 *>(wtl + pl * WS + 8 * sub) = make_uint4(
+            lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16));
+      }
+    } else {
+      *reinterpret_cast<float4 *>(wt + pl * WS + 8 * sub) = make_float4(wv[0], wv[1], wv[2], wv[3]);
+      *reinterpret_cast<float4 *>(wt + pl * WS + 8 * sub + 4) = make_float4(wv[4], wv[5], wv[6], wv[7]);
+    }
+    __syncthreads();
+    // contraction on the matrix cores
+    const int ar = 16 * wave + (lane & 15), kg = lane >> 4;
+    if constexpr (BF) {
+      const v8bf a = *reinterpret_cast<const v8bf *>(wt + ar * WS + 8 * kg);
+      v8bf al;
+      if constexpr (SPLIT) al = *reinterpret_cast<const v8bf *>(wtl + ar * WS + 8 * kg);
+#pragma unroll
+      for (int cb = 0; cb < NB; cb++) {
+        const int br_ = (16 * cb + (lane & 15)) * WS + 8 * kg;
+        const v8bf b = *reinterpret_cast<const v8bf *>(dch + br_);
+        if constexpr (SPLIT) {
+          const v8bf bl = *reinterpret_cast<const v8bf *>(dchl + br_);
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[cb], 0, 0, 0);
+        }
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[cb], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < kFacetChunk / 4; ks++) {
+        const float a = wt[ar * WS + 4 * ks + kg];
+#pragma unroll
+        for (int cb = 0; cb < NB; cb++) {
+          const float b = dch[(16 * cb + (lane & 15)) * WS + 4 * ks + kg];
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  {
+    float *rp = red + ((size_t)sub * kTilePts + pl) * 9;
+    rp[0] = br;
+    rp[1] = v0;
+    rp[2] = v1;
+    rp[3] = v2;
+    rp[4] = u0;
+    rp[5] = u1;
+    rp[6] = u2;
+    rp[7] = scl;
+    rp[8] = scls;
+  }
+  __syncthreads();
+  const int ld = P.npad;
+  double *J = mem + M.J;                       // row ma (ymod) in FP64
+  float *Jf = reinterpret_cast<float *>(J);    // rows 0..ma-1 in FP32
+  if (tid < kTilePts && p0 + tid < P.n_points) {
+    const int p = p0 + tid;
+    double t[9];
+    for (int k = 0; k < 9; k++) t[k] = 0;
+    for (int q = 0; q < kWaves; q++)
+      for (int k = 0; k < 9; k++) t[k] += (double)red[((size_t)q * kTilePts + tid) * 9 + k];
+    const PointGeom G = gst[tid];
+    const double sc = G.scale;
+    const double V[3] = {t[1], t[2], t[3]}, U[3] = {t[4], t[5], t[6]};
+    double r[3];
+    rotation_derivs(P, p, blm, G, V, U, r);
+    for (int k = 0; k < 3; k++) Jf[(size_t)(P.ncoef + k) * ld + p] = (float)(sc * r[k]);
+    for (int i = 0; i < 3; i++) Jf[(size_t)(P.ncoef + 3 + i) * ld + p] = (float)(t[0] * G.dphp[i]);
+    Jf[(size_t)(P.ncoef + 6) * ld + p] = (float)(sc * t[7] * cld);
+    Jf[(size_t)(P.ncoef + 7) * ld + p] = (float)(sc * t[8]);
+    J[(size_t)P.ma * ld + p] = t[0] * sc;
+  }
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) {
+    const int c = 16 * cb + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int pp = 16 * wave + 4 * (lane >> 4) + r;
+      const int p = p0 + pp;
+      if (p < P.n_points && c < P.ncoef) Jf[(size_t)c * ld + p] = (float)(gst[pp].scale * (double)acc[cb][r]);
+    }
+  }
+}
+
 // neq_kernel: one workgroup per slot.  Convexity rows, normal equations at
 // the trial parameters, then accept/reject and the driver bookkeeping.
-// LDS: staging [2*kChunk*mfit + 3*kChunk], area/g [F], dave [ma+1], x [ma],
-// scalars.
-template <int NT>
+// LDS: staged rows (neq_jn_doubles), aux [3*kNeqChunk], dave [ma+1], x [ma],
+// scalars [4], dave/ave [ma].
+template <int NT, bool MIX>
 __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
@@ -1204,24 +1352,19 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) 
   const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
   const SlotMem M = slot_mem(ma, mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
-  const int MS = (mfit + 15) / 16 * 16 + 1;
-  double *wst = sm, *area = wst + (size_t)kNeqChunk * (MS + 3), *g = area + P.nfac, *dave = g + P.nfac;
+  double *wst = sm, *dave = wst + neq_jn_doubles(mfit, MIX) + 3 * kNeqChunk;
   double *x = dave + kMaxMa + 1, *scal = x + kMaxMa;  // scal[0] = chi2, [1..3] = chck
+  double *drat = scal + 4;                            // [ma] dave/ave (mixed forms)
   const int eval = S.eval;
   const int sel = S.sel;
   double *atr = mem + (sel ? M.alpha0 : M.alpha1), *btr = mem + (sel ? M.beta0 : M.beta1);
   if (eval) {
     if (tid == 0) atomicAdd(W.done + 1, 1);
     for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
-    for (int f = tid; f < P.nfac; f += kThreads) {
-      area[f] = mem[M.area + f];
-      g[f] = mem[M.g + f];
-    }
-    __syncthreads();
-    conv_rows(P, area, g, mem + M.J, scal + 1);
+    conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1);
     __threadfence_block();
     __syncthreads();
-    normal_equations_mfma<NT>(P, mem + M.J, W.conw_r, wst, dave, btr, scal, atr);
+    normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_r, wst, dave, drat, btr, scal, atr);
     __threadfence_block();
     __syncthreads();
   }
@@ -1368,31 +1511,29 @@ __global__ void make_items(double freq_start, double freq_step, int n_from, int 
 
 using ModelFn = void (*)(DevProblem, DevWork);
 
-template <bool F32>
 ModelFn pick_model(int ncp) {
   switch (ncp) {
-    case 8: return model_kernel<8, F32>;
-    case 16: return model_kernel<16, F32>;
-    case 24: return model_kernel<24, F32>;
-    case 32: return model_kernel<32, F32>;
-    case 40: return model_kernel<40, F32>;
-    case 48: return model_kernel<48, F32>;
-    case 56: return model_kernel<56, F32>;
-    case 64: return model_kernel<64, F32>;
-    case 72: return model_kernel<72, F32>;
-    case 80: return model_kernel<80, F32>;
-    case 88: return model_kernel<88, F32>;
-    case 96: return model_kernel<96, F32>;
-    case 104: return model_kernel<104, F32>;
-    case 112: return model_kernel<112, F32>;
-    case 120: return model_kernel<120, F32>;
-    case 128: return model_kernel<128, F32>;
+    case 8: return model_kernel<8>;
+    case 16: return model_kernel<16>;
+    case 24: return model_kernel<24>;
+    case 32: return model_kernel<32>;
+    case 40: return model_kernel<40>;
+    case 48: return model_kernel<48>;
+    case 56: return model_kernel<56>;
+    case 64: return model_kernel<64>;
+    case 72: return model_kernel<72>;
+    case 80: return model_kernel<80>;
+    case 88: return model_kernel<88>;
+    case 96: return model_kernel<96>;
+    case 104: return model_kernel<104>;
+    case 112: return model_kernel<112>;
+    case 120: return model_kernel<120>;
+    case 128: return model_kernel<128>;
     default: return nullptr;
   }
 }
 
-using MfmaFn = void (*)(DevProblem, DevWork);
-MfmaFn pick_mfma(int ncoef) {
+ModelFn pick_mfma(int ncoef) {
   switch ((ncoef + 15) / 16) {
     case 1: return model_mfma_kernel<1>;
     case 2: return model_mfma_kernel<2>;
@@ -1402,6 +1543,21 @@ MfmaFn pick_mfma(int ncoef) {
     case 6: return model_mfma_kernel<6>;
     case 7: return model_mfma_kernel<7>;
     case 8: return model_mfma_kernel<8>;
+    default: return nullptr;
+  }
+}
+
+template <int MODE>
+ModelFn pick_mixed(int ncoef) {
+  switch ((ncoef + 15) / 16) {
+    case 1: return model_mixed_kernel<1, MODE>;
+    case 2: return model_mixed_kernel<2, MODE>;
+    case 3: return model_mixed_kernel<3, MODE>;
+    case 4: return model_mixed_kernel<4, MODE>;
+    case 5: return model_mixed_kernel<5, MODE>;
+    case 6: return model_mixed_kernel<6, MODE>;
+    case 7: return model_mixed_kernel<7, MODE>;
+    case 8: return model_mixed_kernel<8, MODE>;
     default: return nullptr;
   }
 }
@@ -1420,18 +1576,18 @@ size_t step_lds(int mfit, int nfac) {
 size_t model_lds(int nfac) {
   return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
 }
-size_t neq_lds(int mfit, int nfac) {
-  const int MS = (mfit + 15) / 16 * 16 + 1;
-  return ((size_t)kNeqChunk * (MS + 3) + 2 * (size_t)nfac + 2 * kMaxMa + 1 + 8) * sizeof(double);
+size_t neq_lds(int mfit, bool mix) {
+  return ((size_t)neq_jn_doubles(mfit, mix) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8) * sizeof(double);
 }
 
 using NeqFn = void (*)(DevProblem, DevWork);
+template <bool MIX>
 NeqFn pick_neq(int mfit) {
   const int nb = (mfit + 15) / 16, ntile = nb * (nb + 1) / 2, nt = (ntile + 3) / 4;
-  if (nt <= 1) return neq_kernel<1>;
-  if (nt <= 3) return neq_kernel<3>;
-  if (nt <= 6) return neq_kernel<6>;
-  if (nt <= 9) return neq_kernel<9>;
+  if (nt <= 1) return neq_kernel<1, MIX>;
+  if (nt <= 3) return neq_kernel<3, MIX>;
+  if (nt <= 6) return neq_kernel<6, MIX>;
+  if (nt <= 9) return neq_kernel<9, MIX>;
   return nullptr;
 }
 
@@ -1439,8 +1595,9 @@ NeqFn pick_neq(int mfit) {
 
 struct ps_engine {
   int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;  // second stream: second slot group
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evg = nullptr;
+  int groups = 1;
   std::string err;
   int precision = PS_PREC_FP64;
   bool have_problem = false;
@@ -1556,38 +1713,63 @@ int run_batch(ps_engine *e, DevWork W) {
   hipLaunchKernelGGL(reset_batch, dim3((nslots + 255) / 256), dim3(256), 0, e->stream, e->st, nslots, W.queue,
                      W.done);
   HIP_TRY(hipGetLastError());
+  // The slots are split into groups, each driven on its own stream: the
+  // kernels of different groups are independent (they share only the item
+  // queue and the done counters, both atomic), so the latency-bound
+  // per-slot kernels of one group overlap the model kernel of the other.
+  const int ng = (e->groups > 1 && nslots >= 2 * kWaves) ? 2 : 1;
+  DevWork Wg[2] = {W, W};
+  hipStream_t sg[2] = {e->stream, e->stream2};
+  Wg[0].nslots = ng == 2 ? (nslots + 1) / 2 : nslots;
+  if (ng == 2) {
+    Wg[1].nslots = nslots - Wg[0].nslots;
+    Wg[1].st = e->st + Wg[0].nslots;
+    Wg[1].mem = e->mem + (size_t)Wg[0].nslots * e->slot_doubles;
+    HIP_TRY(hipEventRecord(e->evg, e->stream));
+    HIP_TRY(hipStreamWaitEvent(e->stream2, e->evg, 0));
+  }
   const int per_fit = (P.n_iter_max > P.n_iter_min ? P.n_iter_max : P.n_iter_min) + 3;
   const long long rounds = (W.n_items + nslots - 1) / nslots + 1;
   const long long guard = rounds * per_fit * 2 + 64;
   const int poll = 8;
-  if (e->profile && e->pev.empty()) {
-    e->pev.resize(2 * poll);
+  if (e->profile && e->pev.size() < (size_t)(4 * poll)) {
+    for (auto ev : e->pev) (void)hipEventDestroy(ev);
+    e->pev.assign(4 * poll, nullptr);
     for (auto &ev : e->pev) HIP_TRY(hipEventCreate(&ev));
   }
   HIP_TRY(hipEventRecord(e->ev0, e->stream));
   long long it = 0;
   while (true) {
     for (int k = 0; k < poll; k++) {
-      hipLaunchKernelGGL(step_kernel, dim3(nslots), dim3(kStepThreads), e->lds_step, e->stream, P, W);
-      hipLaunchKernelGGL(curv_kernel, dim3(nslots * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                         e->stream, P, W);
-      if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k], e->stream));
-      hipLaunchKernelGGL(e->model, dim3(nslots * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads),
-                         e->lds_model, e->stream, P, W);
-      if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * k + 1], e->stream));
-      hipLaunchKernelGGL(e->neq, dim3(nslots), dim3(kThreads), e->lds_neq, e->stream, P, W);
+      for (int g = 0; g < ng; g++) {
+        const DevWork &Wk = Wg[g];
+        const int ns = Wk.nslots;
+        hipLaunchKernelGGL(step_kernel, dim3(ns), dim3(kStepThreads), e->lds_step, sg[g], P, Wk);
+        hipLaunchKernelGGL(curv_kernel, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g],
+                           P, Wk);
+        if (e->profile) HIP_TRY(hipEventRecord(e->pev[4 * k + 2 * g], sg[g]));
+        hipLaunchKernelGGL(e->model, dim3(ns * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads), e->lds_model,
+                           sg[g], P, Wk);
+        if (e->profile) HIP_TRY(hipEventRecord(e->pev[4 * k + 2 * g + 1], sg[g]));
+        hipLaunchKernelGGL(e->neq, dim3(ns), dim3(kThreads), e->lds_neq, sg[g], P, Wk);
+      }
     }
     HIP_TRY(hipGetLastError());
     it += poll;
+    if (ng == 2) {
+      HIP_TRY(hipEventRecord(e->evg, e->stream2));
+      HIP_TRY(hipStreamWaitEvent(e->stream, e->evg, 0));
+    }
     HIP_TRY(hipMemcpyAsync(e->host_done, W.done, 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->profile) {
-      for (int k = 0; k < poll; k++) {
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * k], e->pev[2 * k + 1]));
-        e->model_ms += ms;
-      }
-      e->model_launches += poll;
+      for (int k = 0; k < poll; k++)
+        for (int g = 0; g < ng; g++) {
+          float ms = 0;
+          HIP_TRY(hipEventElapsedTime(&ms, e->pev[4 * k + 2 * g], e->pev[4 * k + 2 * g + 1]));
+          e->model_ms += ms;
+        }
+      e->model_launches += poll * ng;
     }
     if (*e->host_done >= W.n_items) break;
     if (it > guard) {
@@ -1628,7 +1810,9 @@ int ps_create(int device, ps_engine **out) {
   auto *e = new ps_engine();
   e->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&e->evg, hipEventDisableTiming) != hipSuccess ||
       hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     delete e;
@@ -1640,6 +1824,7 @@ int ps_create(int device, ps_engine **out) {
     return PS_ERR_HIP;
   }
   e->cu_count = prop.multiProcessorCount;
+  if (const char *g = getenv("PSAMD_GROUPS")) e->groups = atoi(g) > 1 ? 2 : 1;
   *out = e;
   return PS_OK;
 }
@@ -1648,6 +1833,7 @@ int ps_destroy(ps_engine *e) {
   if (!e) return PS_ERR_ARG;
   (void)hipSetDevice(e->device);
   (void)hipStreamSynchronize(e->stream);
+  if (e->stream2) (void)hipStreamSynchronize(e->stream2);
   free_problem(e);
   void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows,
                   e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
@@ -1657,7 +1843,9 @@ int ps_destroy(ps_engine *e) {
   for (auto ev : e->pev) (void)hipEventDestroy(ev);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->evg) (void)hipEventDestroy(e->evg);
   if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->stream2) (void)hipStreamDestroy(e->stream2);
   delete e;
   return PS_OK;
 }
@@ -1666,7 +1854,7 @@ const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "nul
 
 int ps_set_precision(ps_engine *e, int precision) {
   if (!e) return PS_ERR_ARG;
-  if (precision != PS_PREC_FP64 && precision != PS_PREC_FP32) {
+  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16X3) {
     e->err = "unknown precision";
     return PS_ERR_ARG;
   }
@@ -1733,10 +1921,13 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
     return PS_ERR_LIMIT;
   }
   const int ncp = (ncoef + 7) / 8 * 8;
+  const int nb = (ncoef + 15) / 16;
+  const bool mixed = e->precision != PS_PREC_FP64;
+  const int mode = e->precision == PS_PREC_BF16 ? 1 : (e->precision == PS_PREC_BF16X3 ? 2 : 0);
   const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs)
-  e->use_mfma = e->precision == PS_PREC_FP64 && !(kv && !strcmp(kv, "valu"));
-  ModelFn mk = e->precision == PS_PREC_FP32 ? pick_model<true>(ncp) : pick_model<false>(ncp);
-  if (e->use_mfma) mk = pick_mfma(ncoef);
+  e->use_mfma = mixed || !(kv && !strcmp(kv, "valu"));
+  ModelFn mk = !mixed ? (e->use_mfma ? pick_mfma(ncoef) : pick_model(ncp))
+                      : (mode == 1 ? pick_mixed<1>(ncoef) : (mode == 2 ? pick_mixed<2>(ncoef) : pick_mixed<0>(ncoef)));
   if (!mk) {
     e->err = "no model kernel for this coefficient count";
     return PS_ERR_LIMIT;
@@ -1760,18 +1951,34 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   lc_start[p->n_lc + 1] = N + 3;
   lc_inrel[p->n_lc] = 0;
   std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
-  std::vector<float> nxf(F), nyf(F), nzf(F), dsphf((size_t)F * ncp, 0.0f);
   std::vector<double4> nor4(F);
+  std::vector<float4> nor4f(F);
+  const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk, ncT = 16 * nb;
+  std::vector<float> dsphT32((size_t)ncT * fpad, 0.0f);
+  std::vector<unsigned short> dsphT16((size_t)ncT * fpad, 0), dsphT16lo((size_t)ncT * fpad, 0);
+  auto to_bf16 = [](float v) {
+    unsigned u;
+    memcpy(&u, &v, 4);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  };
   for (int f = 0; f < F; f++) {
     nx[f] = p->nor[3 * f];
     ny[f] = p->nor[3 * f + 1];
     nz[f] = p->nor[3 * f + 2];
-    for (int c = 0; c < ncoef; c++) dsph[(size_t)f * ncp + c] = p->dsph[(size_t)f * ncoef + c];
+    for (int c = 0; c < ncoef; c++) {
+      const double v = p->dsph[(size_t)f * ncoef + c];
+      dsph[(size_t)f * ncp + c] = v;
+      const float vf = (float)v;
+      dsphT32[(size_t)c * fpad + f] = vf;
+      const unsigned short hi = to_bf16(vf);
+      const unsigned hb = (unsigned)hi << 16;
+      float hf;
+      memcpy(&hf, &hb, 4);
+      dsphT16[(size_t)c * fpad + f] = hi;
+      dsphT16lo[(size_t)c * fpad + f] = to_bf16(vf - hf);
+    }
     nor4[f] = make_double4(nx[f], ny[f], nz[f], 0.0);
-    nxf[f] = (float)nx[f];
-    nyf[f] = (float)ny[f];
-    nzf[f] = (float)nz[f];
-    for (int c = 0; c < ncp; c++) dsphf[(size_t)f * ncp + c] = (float)dsph[(size_t)f * ncp + c];
+    nor4f[f] = make_float4((float)nx[f], (float)ny[f], (float)nz[f], 0.0f);
   }
   const int L1 = p->lmax + 1, M1 = p->mmax + 1;
   std::vector<double> cpleg((size_t)ngrp * F), cfc((size_t)M1 * F), cfs((size_t)M1 * F);
@@ -1807,10 +2014,10 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   UP(p->darea, F, darea);
   UP(dsph.data(), dsph.size(), dsph);
   UP(nor4.data(), F, nor4);
-  UP(nxf.data(), F, nxf);
-  UP(nyf.data(), F, nyf);
-  UP(nzf.data(), F, nzf);
-  UP(dsphf.data(), dsphf.size(), dsphf);
+  UP(nor4f.data(), F, nor4f);
+  UP(dsphT32.data(), dsphT32.size(), dsphT32);
+  UP(dsphT16.data(), dsphT16.size(), dsphT16);
+  UP(dsphT16lo.data(), dsphT16lo.size(), dsphT16lo);
   UP(cpleg.data(), cpleg.size(), cpleg);
   UP(cfc.data(), cfc.size(), cfc);
   UP(cfs.data(), cfs.size(), cfs);
@@ -1826,6 +2033,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   D.ngrp = ngrp;
   D.ncoef = ncoef;
   D.ncp = ncp;
+  D.fpad = fpad;
   D.ma = ma;
   D.mfit = mfit;
   D.phi0 = p->phi0;
@@ -1855,8 +2063,8 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   e->n_poles = p->n_poles;
   e->model = mk;
   e->lds_step = step_lds(mfit, F);
-  e->lds_model = e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F);
-  e->lds_neq = neq_lds(mfit, F);
+  e->lds_model = mixed ? mixed_layout(F, nb, mode).total : (e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
+  e->lds_neq = neq_lds(mfit, mixed);
   const size_t lds_max = 160 * 1024;
   if (e->lds_step > lds_max || e->lds_neq > lds_max || e->lds_model > lds_max) {
     e->err = "problem needs more LDS than a CU has";
@@ -1864,7 +2072,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   }
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(step_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_step));
-  e->neq = pick_neq(mfit);
+  e->neq = mixed ? pick_neq<true>(mfit) : pick_neq<false>(mfit);
   if (!e->neq) {
     e->err = "too many free parameters";
     return PS_ERR_LIMIT;

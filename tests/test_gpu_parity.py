@@ -8,10 +8,20 @@ Tolerances:
     amplify last-bit differences on a few trajectories -- and the fitted
     period within 1e-7;
   * output rows, FP64: period 1e-9, rms and chi2 1e-6, pole within 0.01 deg;
-  * reduced-precision form (FP32 model stage): one evaluation within 1e-5;
-    over a window of trial periods around the true period the chi2 minimum
-    within 1e-4 relative of the FP64 result, at the same trial period
-    (at most one scan step away).
+  * mixed forms (FP32 facet stage, contraction and alpha on the matrix cores
+    with FP32 accumulation; FP64 residuals, chi2, beta and solve): one
+    evaluation's chi2 within 1e-5 and alpha/beta within the form's bound
+    relative to max|alpha|, max|beta| (FP32 operands 1e-5/1e-4, split bf16
+    1e-4/2e-3, plain bf16 1e-2/1e-1); over a window of trial periods around
+    the true period, the best row's fitted period within one scan step
+    (P^2 * frequency step) of the FP64 best row's, and its chi2 within 1e-4
+    relative of the FP64 minimum (FP32, split bf16).  Plain bf16 operands
+    (8 significant bits) do not meet the chi2 bar: its minimum is held to
+    1e-3, and the split form is the bf16 form to use.  (Near the minimum the
+    chi2 of neighbouring trial periods differs by less than the 1e-5 spread
+    that a different rounding of 50 LM iterations gives, so which trial
+    period's fit is lowest is not itself a stable quantity; the fitted
+    period is.)
 Fit-level tests use the asteroid on a 10-deg inclined orbit: on a coplanar
 orbit the poles lambda and lambda+180 deg fit equally well, and which of two
 equal minima a fit reaches is decided by last-bit noise.
@@ -161,42 +171,53 @@ def test_reference_sample_rows(golden_dir):
     engine.close()
 
 
-def test_fp32_eval_within_tolerance(small_input):
-    from periodsearch_amd import PS_PREC_FP32, Engine
+MIXED = [("fp32", 1, 1e-5, 1e-4), ("bf16", 2, 1e-2, 1e-1), ("bf16x3", 3, 1e-4, 2e-3)]
+MIXED_MIN = [("fp32", 1, 1e-4), ("bf16", 2, 1e-3), ("bf16x3", 3, 1e-4)]
+
+
+@pytest.mark.parametrize("name,prec,tol_alpha,tol_beta", MIXED, ids=[m[0] for m in MIXED])
+def test_mixed_eval_within_tolerance(small_input, name, prec, tol_alpha, tol_beta):
+    from periodsearch_amd import Engine
 
     eng = Engine(0)
-    inp, orc = _setup(small_input, eng, PS_PREC_FP32)
+    inp, orc = _setup(small_input, eng, prec)
     cgs = _eval_cases(inp, orc)
     alpha, beta, chi, _ = eng.eval_mrqcof(np.array([c[1:] for c in cgs]), inp.conw_r0)
+    worst = [0.0, 0.0, 0.0]
     for i, cg1 in enumerate(cgs):
         A, b, c = orc.mrqcof(cg1)
-        assert np.abs(alpha[i] - A).max() / np.abs(A).max() < 1e-5, i
-        assert np.abs(beta[i] - b).max() / np.abs(b).max() < 1e-4, i
-        assert _rel(chi[i], c) < 1e-5, i
+        ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
+        eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+        ec = float(_rel(chi[i], c))
+        worst = [max(worst[0], ea), max(worst[1], eb), max(worst[2], ec)]
+        assert ea < tol_alpha and eb < tol_beta and ec < 1e-5, (i, ea, eb, ec)
+    print(f"{name}: alpha {worst[0]:.2e} beta {worst[1]:.2e} chi2 {worst[2]:.2e} (relative to max, worst of {len(cgs)})")
     eng.close()
 
 
-def test_fp32_chi2_minimum_matches_fp64(tmp_path):
+@pytest.mark.parametrize("name,prec,tol_chi2", MIXED_MIN, ids=[m[0] for m in MIXED_MIN])
+def test_mixed_chi2_minimum_matches_fp64(tmp_path, name, prec, tol_chi2):
     """Reduced precision judged where it matters: over the trial periods
     around the true 7.3 h period, the row with the smallest chi2 and its chi2."""
-    from periodsearch_amd import PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
+    from periodsearch_amd import PS_PREC_FP64, Engine, Input, synth
 
     path = str(tmp_path / "period_search_in")
     synth.write_workload("small", path, incl_deg=10.0, per_start=7.296, per_end=7.304)
     inp = Input(path)
     best = {}
-    for prec in (PS_PREC_FP64, PS_PREC_FP32):
+    for pr in (PS_PREC_FP64, prec):
         eng = Engine(0)
-        eng.set_precision(prec)
+        eng.set_precision(pr)
         eng.set_problem(inp.problem, keep=inp)
         rows = eng.scan(inp.freq_start, inp.freq_step, 1, inp.n_periods, inp.conw_r0)
         k = int(np.argmin(rows["chisq"]))
-        best[prec] = (k, rows["chisq"][k], rows["period_h"][k])
+        best[pr] = (k, rows["chisq"][k], rows["period_h"][k])
         eng.close()
-    (k64, c64, p64), (k32, c32, p32) = best[PS_PREC_FP64], best[PS_PREC_FP32]
-    print(f"chi2 minimum: FP64 row {k64} chi2 {c64:.9g} P {p64:.8f} h; FP32 row {k32} chi2 {c32:.9g} P {p32:.8f} h")
-    assert abs(k64 - k32) <= 1
-    assert _rel(c32, c64) < 1e-4
+    (k64, c64, p64), (kx, cx, px) = best[PS_PREC_FP64], best[prec]
+    print(f"chi2 minimum: FP64 row {k64} chi2 {c64:.9g} P {p64:.8f} h; {name} row {kx} chi2 {cx:.9g} P {px:.8f} h")
+    step_h = p64 * p64 * inp.freq_step  # one scan step in period at P
+    assert abs(px - p64) <= step_h, (px, p64, step_h)
+    assert _rel(cx, c64) < tol_chi2
     import oracle
 
     orc = oracle.Oracle(path)
