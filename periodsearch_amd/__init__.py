@@ -1,4 +1,5 @@
-"""MI355X period-search engine: Python mirror of the C ABI in include/periodsearch.h.
+"""MI355X period-search engine: Python mirror of the C ABI in include/periodsearch.h
+and include/ps_proposal.h.
 
 The compute path is the HIP library ``libpsamd.so`` built in this directory
 (``python -m periodsearch_amd.build``).  Importing this package loads it; if it
@@ -14,6 +15,9 @@ The classes follow the roles of the reference's CUDA backend
     Engine.scan(...)          CUDAStart (rows as numbers; text via format_row)
     Engine.fit_batch(...)     per-fit results (test seam)
     Engine.eval_mrqcof(...)   one normal-equation build (test seam)
+
+Learned starting points (no counterpart in the reference): Engine.lc_features,
+Engine.fit_batch_from and the Mlp network (see periodsearch_amd.proposal).
 """
 from __future__ import annotations
 
@@ -34,6 +38,7 @@ PS_PREC_BF16X3 = 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 
 class PsProblem(C.Structure):
@@ -92,6 +97,17 @@ def _load() -> C.CDLL:
         "ps_format_row": (C.c_int, [C.POINTER(PsRow), C.c_int, C.c_char_p, C.c_int]),
         "ps_shape_basis": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "ps_lc_features": (C.c_int, [vp, _dp, C.c_int, C.c_int, _fp]),
+        "ps_fit_batch_from": (C.c_int, [vp, _dp, C.c_int, C.c_double, vp, _dp]),
+        "ps_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+        "ps_mlp_destroy": (C.c_int, [vp]),
+        "ps_mlp_last_error": (C.c_char_p, [vp]),
+        "ps_mlp_param_count": (C.c_longlong, [vp]),
+        "ps_mlp_set_params": (C.c_int, [vp, _fp, C.c_longlong]),
+        "ps_mlp_get_params": (C.c_int, [vp, _fp, C.c_longlong]),
+        "ps_mlp_forward": (C.c_int, [vp, _fp, C.c_int, _fp]),
+        "ps_mlp_gradients": (C.c_int, [vp, _fp, _fp, _fp, C.c_int, _fp, _fp]),
+        "ps_mlp_train_step": (C.c_int, [vp, _fp, _fp, _fp, C.c_int, C.c_float, _fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -102,11 +118,14 @@ def _load() -> C.CDLL:
 
 lib = _load()
 
-# Symbols declared in include/periodsearch.h (checked by the CPU test suite).
+# Symbols declared in include/*.h (checked by the CPU test suite).
 EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_set_slots", "ps_set_problem",
-            "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms", "ps_set_profiling", "ps_kernel_stats",
-            "ps_input_read", "ps_input_free", "ps_input_problem", "ps_input_scalars", "ps_format_row",
-            "ps_shape_basis")
+            "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
+            "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
+            "ps_input_scalars", "ps_format_row", "ps_shape_basis",
+            "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
+            "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
+            "ps_mlp_train_step")
 
 
 def _d(a: np.ndarray):
@@ -115,6 +134,10 @@ def _d(a: np.ndarray):
 
 def _i(a: np.ndarray):
     return a.ctypes.data_as(_ip)
+
+
+def _f(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
 
 
 class PsError(RuntimeError):
@@ -245,6 +268,26 @@ class Engine:
                     "ps_eval_mrqcof")
         return alpha, beta, chi, chck
 
+    def lc_features(self, freq, nbins: int = 64) -> np.ndarray:
+        """Phase-folded light-curve features at trial frequencies (1/h):
+        [n][nbins + 2] (bins, mean phase angle, mean |Earth latitude|)."""
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        out = np.zeros((len(freq), nbins + 2), dtype=np.float32)
+        self._check(lib.ps_lc_features(self._h, _d(freq), len(freq), nbins, _f(out)), "ps_lc_features")
+        return out
+
+    def fit_batch_from(self, cg: np.ndarray, conw_r: float, want_params: bool = False):
+        """LM fits from given starting parameters cg[n][ma]; with want_params
+        also the final parameters [n][ma]."""
+        cg = np.ascontiguousarray(np.atleast_2d(cg), dtype=np.float64)
+        _, ma, _ = self.dims()
+        assert cg.shape[1] == ma
+        out = np.zeros(cg.shape[0], dtype=FIT_DTYPE)
+        cgo = np.zeros_like(cg) if want_params else None
+        self._check(lib.ps_fit_batch_from(self._h, _d(cg), cg.shape[0], conw_r, out.ctypes.data,
+                                          _d(cgo) if want_params else None), "ps_fit_batch_from")
+        return (out, cgo) if want_params else out
+
     def set_profiling(self, on: bool = True):
         self._check(lib.ps_set_profiling(self._h, 1 if on else 0), "ps_set_profiling")
 
@@ -260,6 +303,63 @@ class Engine:
         n = C.c_longlong()
         lib.ps_last_kernel_ms(self._h, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+
+class Mlp:
+    """The proposal network on the GPU: n_in -> hidden -> hidden -> n_out,
+    ReLU, FP32 (include/ps_proposal.h)."""
+
+    def __init__(self, n_in: int, n_hidden: int, n_out: int, max_batch: int = 1024, device: int = 0):
+        h = C.c_void_p()
+        rc = lib.ps_mlp_create(device, n_in, n_hidden, n_out, max_batch, C.byref(h))
+        if rc != 0:
+            raise PsError(f"ps_mlp_create failed: {rc}")
+        self._h = h
+        self.shape = (n_in, n_hidden, n_out)
+        self.max_batch = max_batch
+        self.n_params = lib.ps_mlp_param_count(h)
+
+    def close(self):
+        if self._h:
+            lib.ps_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise PsError(f"{what} failed ({rc}): {lib.ps_mlp_last_error(self._h).decode()}")
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(lib.ps_mlp_set_params(self._h, _f(p), p.size), "ps_mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self._check(lib.ps_mlp_get_params(self._h, _f(p), p.size), "ps_mlp_get_params")
+        return p
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.shape[2]), dtype=np.float32)
+        self._check(lib.ps_mlp_forward(self._h, _f(x), x.shape[0], _f(y)), "ps_mlp_forward")
+        return y
+
+    def gradients(self, x, t, w):
+        x, t, w = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, w))
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib.ps_mlp_gradients(self._h, _f(x), _f(t), _f(w), x.shape[0], _f(g), C.byref(loss)),
+                    "ps_mlp_gradients")
+        return g, loss.value
+
+    def train_step(self, x, t, w, lr: float) -> float:
+        x, t, w = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, w))
+        loss = C.c_float()
+        self._check(lib.ps_mlp_train_step(self._h, _f(x), _f(t), _f(w), x.shape[0], lr, C.byref(loss)),
+                    "ps_mlp_train_step")
+        return loss.value
 
 
 def format_row(row, round_pole: bool = False) -> str:

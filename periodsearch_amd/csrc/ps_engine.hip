@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "periodsearch.h"
+#include "ps_proposal.h"
 
 namespace {
 
@@ -70,6 +71,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8))); // bf16 operand of v_mf
 struct DevProblem {
   // points, SoA, real points only
   const double *tim, *y, *sig, *w, *ex, *ey, *ez, *sx, *sy, *sz;
+  const float *ynorm;   // [n_points] brightness / light-curve mean - 1 (proposal features)
   const int *lc_start;  // [n_lc + 2]: real light curves then the conv "light curve"
   const int *lc_inrel;  // [n_lc + 1]
   int n_points, n_lc;
@@ -136,12 +138,13 @@ __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
 
 // One batch of work: fits (mode 0) or single model evaluations (mode 1).
 struct DevWork {
-  int n_items, mode, nslots, pad;
+  int n_items, mode, nslots, pad;  // mode 0: fits from the reference starts, 1: evaluations, 2: fits from cg_in
   const double *freq;       // [n_items] (mode 0)
   const int *pole;          // [n_items] (mode 0)
-  const double *cg_in;      // [n_items][ma] (mode 1)
+  const double *cg_in;      // [n_items][ma] (modes 1, 2)
   double conw_r;
-  ps_fit_out *out;          // [n_items] (mode 0)
+  ps_fit_out *out;          // [n_items] (modes 0, 2)
+  double *cg_out;           // [n_items][ma] final parameters, or null
   double *ev_alpha, *ev_beta, *ev_chi, *ev_chck;  // (mode 1)
   int *queue, *done;   // done[1] counts model evaluations
   SlotState *st;
@@ -826,6 +829,8 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
     // per-fit summary (period_search_BOINC.cpp:900-911); totarea from the
     // shape of the last curv() call.
     eval_curv(P, mem + M.alast, area, g);
+    if (W.cg_out)
+      for (int l = tid; l < ma; l += kStepThreads) W.cg_out[(size_t)s_item * ma + l] = mem[M.a + l];
     __syncthreads();
     if (tid == 0) {
 #pragma clang fp contract(off)
@@ -866,7 +871,7 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
       return;
     }
     double *a = mem + M.a, *atry = mem + M.atry;
-    if (W.mode == 1) {
+    if (W.mode != 0) {  // given starting parameters (evaluation or fit)
       for (int l = tid; l < ma; l += kStepThreads) a[l] = W.cg_in[(size_t)item * ma + l];
     } else {
       init_params(P, W.freq[item], W.pole[item], a);
@@ -1498,6 +1503,36 @@ __global__ void reduce_poles(const ps_fit_out *fits, int n_periods, int n_poles,
   rows[n] = r;
 }
 
+// Proposal features: one workgroup per trial frequency; the points are
+// folded at phase frac(24 freq t) into nbins bins whose sums of ynorm are
+// kept in 2^16 fixed point (integer LDS atomics: the result does not depend
+// on the order of the additions).
+__global__ void __launch_bounds__(kThreads) lc_features(DevProblem P, const double *freq, int nbins, float c_alpha,
+                                                         float c_aspect, float *out) {
+  __shared__ int sum[256], cnt[256];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  for (int b = tid; b < nbins; b += kThreads) {
+    sum[b] = 0;
+    cnt[b] = 0;
+  }
+  __syncthreads();
+  const double fd = 24.0 * freq[i];  // rotations per day
+  for (int p = tid; p < P.n_points; p += kThreads) {
+    const double ph = P.tim[p] * fd;
+    const double fr = ph - floor(ph);
+    const int b = min((int)(fr * nbins), nbins - 1);
+    atomicAdd(&sum[b], __float2int_rn(P.ynorm[p] * 65536.0f));
+    atomicAdd(&cnt[b], 1);
+  }
+  __syncthreads();
+  float *o = out + (size_t)i * (nbins + 2);
+  for (int b = tid; b < nbins; b += kThreads) o[b] = cnt[b] ? (float)sum[b] / 65536.0f / (float)cnt[b] : 0.0f;
+  if (tid == 0) {
+    o[nbins] = c_alpha;
+    o[nbins + 1] = c_aspect;
+  }
+}
+
 // Trial frequencies and poles of periods n_from.. (period_search_BOINC.cpp:812, 817).
 __global__ void make_items(double freq_start, double freq_step, int n_from, int n_periods, int n_poles,
                            double *freq, int *pole) {
@@ -1604,6 +1639,9 @@ struct ps_engine {
   DevProblem P{};
   std::vector<void *> prob_allocs;
   int n_poles = 0, cu_count = 0, max_slots = 2048;
+  float feat_alpha = 0, feat_aspect = 0;  // mean solar phase angle, mean |Earth latitude| [rad]
+  float *d_feat = nullptr;
+  size_t feat_cap = 0;
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
@@ -1622,7 +1660,8 @@ struct ps_engine {
   ps_row *d_rows = nullptr;
   size_t item_cap = 0, row_cap = 0;
   double *d_cg = nullptr, *d_ea = nullptr, *d_eb = nullptr, *d_ec = nullptr, *d_ek = nullptr;
-  size_t ev_cap = 0;
+  size_t ev_cap = 0, cg_cap = 0, cgo_cap = 0;
+  double *d_cgo = nullptr;
   double last_ms = 0;
   long long last_items = 0, last_iters = 0;
   // optional per-launch timing of the model kernel (HIP events on the
@@ -1649,6 +1688,16 @@ void free_problem(ps_engine *e) {
   for (void *p : e->prob_allocs) (void)hipFree(p);
   e->prob_allocs.clear();
   e->have_problem = false;
+}
+
+int ensure_cg(ps_engine *e, size_t n) {
+  if (n <= e->cg_cap) return PS_OK;
+  if (e->d_cg) (void)hipFree(e->d_cg);
+  e->d_cg = nullptr;
+  e->cg_cap = 0;
+  HIP_TRY(hipMalloc(&e->d_cg, n * kMaxMa * sizeof(double)));
+  e->cg_cap = n;
+  return PS_OK;
 }
 
 int ensure_items(ps_engine *e, size_t n) {
@@ -1835,7 +1884,7 @@ int ps_destroy(ps_engine *e) {
   (void)hipStreamSynchronize(e->stream);
   if (e->stream2) (void)hipStreamSynchronize(e->stream2);
   free_problem(e);
-  void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows,
+  void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows, e->d_feat, e->d_cgo,
                   e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -1950,6 +1999,23 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   }
   lc_start[p->n_lc + 1] = N + 3;
   lc_inrel[p->n_lc] = 0;
+  std::vector<float> ynorm(N);
+  double sa = 0, sb = 0;
+  for (int i = 0; i < p->n_lc; i++) {
+    double mean = 0;
+    for (int q = lc_start[i]; q < lc_start[i + 1]; q++) mean += p->brightness[q];
+    mean /= p->lc_points[i];
+    for (int q = lc_start[i]; q < lc_start[i + 1]; q++) ynorm[q] = (float)(p->brightness[q] / mean - 1.0);
+  }
+  for (int q = 0; q < N; q++) {
+    const double *ee = p->ee + 3 * q, *e0 = p->ee0 + 3 * q;
+    double c = ee[0] * e0[0] + ee[1] * e0[1] + ee[2] * e0[2];
+    c = c > 1 ? 1 : (c < -1 ? -1 : c);
+    sa += acos(c);
+    sb += fabs(asin(ee[2] > 1 ? 1 : (ee[2] < -1 ? -1 : ee[2])));
+  }
+  e->feat_alpha = (float)(sa / N);
+  e->feat_aspect = (float)(sb / N);
   std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
   std::vector<double4> nor4(F);
   std::vector<float4> nor4f(F);
@@ -2007,6 +2073,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   UP(sy.data(), N, sy);
   UP(sz.data(), N, sz);
   UP(lc_start.data(), lc_start.size(), lc_start);
+  UP(ynorm.data(), N, ynorm);
   UP(lc_inrel.data(), lc_inrel.size(), lc_inrel);
   UP(nx.data(), F, nx);
   UP(ny.data(), F, ny);
@@ -2200,19 +2267,72 @@ int ps_precalc(ps_engine *e, double freq_start, double freq_end, double freq_ste
   return PS_OK;
 }
 
+int ps_lc_features(ps_engine *e, const double *freq, int n, int nbins, float *out) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (n < 0 || nbins < 1 || nbins > 256 || (n > 0 && (!freq || !out))) return PS_ERR_ARG;
+  if (n == 0) return PS_OK;
+  if ((rc = ensure_items(e, n))) return rc;
+  const size_t need = (size_t)n * (nbins + 2);
+  if (need > e->feat_cap) {
+    if (e->d_feat) (void)hipFree(e->d_feat);
+    e->d_feat = nullptr;
+    e->feat_cap = 0;
+    HIP_TRY(hipMalloc(&e->d_feat, need * sizeof(float)));
+    e->feat_cap = need;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_freq, freq, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  hipLaunchKernelGGL(lc_features, dim3(n), dim3(kThreads), 0, e->stream, e->P, e->d_freq, nbins, e->feat_alpha,
+                     e->feat_aspect, e->d_feat);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, e->d_feat, need * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return PS_OK;
+}
+
+int ps_fit_batch_from(ps_engine *e, const double *cg, int n, double conw_r, ps_fit_out *out, double *cg_out) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!cg || !out))) return PS_ERR_ARG;
+  if (n == 0) return PS_OK;
+  if ((rc = ensure_items(e, n))) return rc;
+  if ((rc = ensure_cg(e, n))) return rc;
+  if (cg_out && (size_t)n > e->cgo_cap) {
+    if (e->d_cgo) (void)hipFree(e->d_cgo);
+    e->d_cgo = nullptr;
+    e->cgo_cap = 0;
+    HIP_TRY(hipMalloc(&e->d_cgo, (size_t)n * kMaxMa * sizeof(double)));
+    e->cgo_cap = n;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_cg, cg, (size_t)n * e->P.ma * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  DevWork W{};
+  W.n_items = n;
+  W.mode = 2;
+  W.cg_in = e->d_cg;
+  W.conw_r = conw_r;
+  W.out = e->d_out;
+  W.cg_out = cg_out ? e->d_cgo : nullptr;
+  if ((rc = run_batch(e, W))) return rc;
+  HIP_TRY(hipMemcpyAsync(out, e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
+  if (cg_out)
+    HIP_TRY(hipMemcpyAsync(cg_out, e->d_cgo, (size_t)n * e->P.ma * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return PS_OK;
+}
+
 int ps_eval_mrqcof(ps_engine *e, const double *cg, int n, double conw_r, double *alpha, double *beta, double *chisq,
                    double *chck) {
   int rc = check_ready(e);
   if (rc) return rc;
   if (n <= 0 || !cg || !alpha || !beta || !chisq || !chck) return PS_ERR_ARG;
   const int ma = e->P.ma, mfit = e->P.mfit;
+  if ((rc = ensure_cg(e, n))) return rc;
   if ((size_t)n > e->ev_cap) {
-    void *bufs[] = {e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
+    void *bufs[] = {e->d_ea, e->d_eb, e->d_ec, e->d_ek};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
-    e->d_cg = e->d_ea = e->d_eb = e->d_ec = e->d_ek = nullptr;
+    e->d_ea = e->d_eb = e->d_ec = e->d_ek = nullptr;
     e->ev_cap = 0;
-    HIP_TRY(hipMalloc(&e->d_cg, (size_t)n * kMaxMa * sizeof(double)));
     HIP_TRY(hipMalloc(&e->d_ea, (size_t)n * kMaxMfit * kMaxMfit * sizeof(double)));
     HIP_TRY(hipMalloc(&e->d_eb, (size_t)n * kMaxMfit * sizeof(double)));
     HIP_TRY(hipMalloc(&e->d_ec, (size_t)n * sizeof(double)));

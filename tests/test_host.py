@@ -9,7 +9,8 @@ import periodsearch_amd as ps
 
 
 def test_every_declared_symbol_is_exported():
-    hdr = open(ps.os.path.join(ps.os.path.dirname(ps.HERE), "include", "periodsearch.h")).read()
+    inc = ps.os.path.join(ps.os.path.dirname(ps.HERE), "include")
+    hdr = "".join(open(ps.os.path.join(inc, f)).read() for f in ("periodsearch.h", "ps_proposal.h"))
     declared = set(re.findall(r"^\w[\w \*]*?\b(ps_\w+)\(", hdr, re.M))
     assert declared == set(ps.EXPORTED)
     for name in declared:
