@@ -12,9 +12,16 @@ The problem is uploaded before the timed region; rows come back to the host
 inside it.  The convexity-weight precalc is excluded (it runs identically in
 any implementation).
 
+Forms: --prec fp64 (plain), fp32 / bf16 / bf16x3 (mixed precision), and
+--form learned (mixed precision with the learned proposal: every trial period
+of the step fitted once from the network's start, the best --candidates
+periods and two neighbours each refitted with the plain ten poles; the value
+is then the plain-equivalent rate, 10 fits per trial period covered, and
+lm_fits_per_s the LM fits actually run).
+
 Prints one JSON line with fits/s, the roofline position of the dominant
 kernel (the model kernel: facet stage + brightness-derivative contraction,
-timed per launch with HIP events on the engine's stream) and the FP64 CPU
+timed per launch with HIP events on the engine's streams) and the FP64 CPU
 oracle timed on this host's cores on a bounded sample.
 """
 from __future__ import annotations
@@ -72,6 +79,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3"), default="fp64")
+    ap.add_argument("--form", choices=("plain", "learned"), default="plain")
+    ap.add_argument("--candidates", type=int, default=4, help="learned form: periods verified per step")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -107,15 +116,28 @@ def main() -> int:
     P = args.periods
     n_next = 1
 
+    lm_fits = 0
+    if args.form == "learned":
+        from periodsearch_amd.proposal import Proposal, scan_learned
+
+        prop = Proposal.load()
+
     def step():
-        nonlocal n_next
-        rows = eng.scan(inp.freq_start, inp.freq_step, n_next, n_next + P - 1, conw_r)
+        nonlocal n_next, lm_fits
+        if args.form == "learned":
+            rows, _, nf, _ = scan_learned(eng, inp, prop, conw_r, n_next, n_next + P - 1,
+                                          n_candidates=args.candidates, halo=2, mirror=False)
+        else:
+            rows = eng.scan(inp.freq_start, inp.freq_step, n_next, n_next + P - 1, conw_r)
+            nf = 10 * P
+        lm_fits += nf
         n_next += P
         return rows
 
     for _ in range(args.warmup):
         step()
     eng.set_profiling(True)
+    lm_fits = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -151,14 +173,20 @@ def main() -> int:
                                "nrows=8 (512 facets), 2-100 h, stop 50, 10 poles",
                    "periods_per_step": P, "fits_per_step": P * 10, "precision": args.prec,
                    "form": {"fp64": "plain FP64", "fp32": "mixed FP32", "bf16": "mixed BF16",
-                            "bf16x3": "mixed split BF16"}[args.prec]},
+                            "bf16x3": "mixed split BF16"}[args.prec] + (" + learned proposal" if args.form == "learned"
+                                                                         else "")},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": "model_kernel", "model_ms_total": round(st["model_ms"], 3),
                      "model_launches": st["model_launches"], "evals": st["evals"],
                      "flop_per_eval": flop_eval},
-        "job_tflops": round(fits * flop_fit / wall / 1e12, 3),
+        "job_tflops": round(lm_fits * flop_fit / wall / 1e12, 3),
+        "lm_fits_per_s": round(lm_fits / wall, 2),
     }
+    if args.form == "learned":
+        out["config"]["learned"] = (f"1 proposed start per trial period; {args.candidates} best periods per step "
+                                    f"and 2 neighbours each refitted from the 10 plain poles; value = 10 x trial "
+                                    f"periods covered per second")
     if not args.no_cpu:
         try:
             ncpu = min(len(os.sched_getaffinity(0)), 16)
