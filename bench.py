@@ -41,6 +41,12 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector = matrix), dense
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix, dense
 BF16_PEAK_TFLOPS = 2500.0  # MI355X bf16 matrix, dense
 WORKLOAD = "wu"
+# HBM bytes per model evaluation of the model kernel on this workload, from the
+# FETCH_SIZE / WRITE_SIZE counter runs (copy-calibrated) summarised in
+# profiles/model_mfma_kernel_fp64_wu1080.md and
+# profiles/model_mixed_kernel_bf16x3_learned_wu1080.md; the mixed forms share
+# the FP32 Jacobian layout.
+MODEL_HBM_BYTES_PER_EVAL = {"fp64": 469.4e3, "fp32": 242.2e3, "bf16": 242.2e3, "bf16x3": 242.2e3}
 
 
 def cpu_baseline(path: str, conw_r: float, n_procs: int, periods_each: int = 1) -> dict:
@@ -71,15 +77,15 @@ def cpu_baseline(path: str, conw_r: float, n_procs: int, periods_each: int = 1) 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--periods", type=int, default=256, help="trial periods per step")
+    ap.add_argument("--periods", type=int, default=1024, help="trial periods per step")
     ap.add_argument("--slots", type=int, default=0, help="resident fits (0 = engine default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
-    ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3"), default="fp64")
-    ap.add_argument("--form", choices=("plain", "learned"), default="plain")
+    ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3"), default="bf16x3")
+    ap.add_argument("--form", choices=("plain", "learned"), default="learned")
     ap.add_argument("--candidates", type=int, default=4, help="learned form: periods verified per step")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -176,7 +182,8 @@ def main() -> int:
                             "bf16x3": "mixed split BF16"}[args.prec] + (" + learned proposal" if args.form == "learned"
                                                                          else "")},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "frac": round(achieved / peak, 4),
+                     "traffic": round(MODEL_HBM_BYTES_PER_EVAL[args.prec] * st["evals"] / max(st["model_launches"], 1)),
                      "kernel": "model_kernel", "model_ms_total": round(st["model_ms"], 3),
                      "model_launches": st["model_launches"], "evals": st["evals"],
                      "flop_per_eval": flop_eval},

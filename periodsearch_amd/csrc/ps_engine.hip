@@ -1632,7 +1632,7 @@ struct ps_engine {
   int device = 0;
   hipStream_t stream = nullptr, stream2 = nullptr;  // second stream: second slot group
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evg = nullptr;
-  int groups = 1;
+  int groups = 2;  // slot groups (streams); PSAMD_GROUPS=1 runs everything on one stream
   std::string err;
   int precision = PS_PREC_FP64;
   bool have_problem = false;
