@@ -51,6 +51,7 @@ constexpr int kChunk = 32;         // points staged per normal-equation chunk
 constexpr int kTilePts = 64;       // points per MFMA model tile (4 row blocks of 16)
 constexpr int kFacetChunk = 32;    // facets per MFMA model chunk (8 k-steps of 4)
 constexpr int kMaxFacets = 1000;
+constexpr int kMaxGroups = 4;      // slot groups, one HIP stream each
 constexpr double kTiny = 1e-8;     // period_search/constants.h:10
 constexpr double kPi = 3.14159265358979323846;
 
@@ -419,7 +420,7 @@ __device__ void conv_rows(const DevProblem &P, const double *area, const double 
     put(tid, n, 0);
     put(tid, n + 1, 0);
     put(tid, n + 2, 0);
-  } else if (tid >= kThreads - 3) {
+  } else if (tid >= kThreads - 3 && tid < kThreads) {
     const int nc = tid - (kThreads - 3);
     const double *nv = nc == 0 ? P.nx : (nc == 1 ? P.ny : P.nz);
     double r = 0;
@@ -1630,9 +1631,10 @@ NeqFn pick_neq(int mfit) {
 
 struct ps_engine {
   int device = 0;
-  hipStream_t stream = nullptr, stream2 = nullptr;  // second stream: second slot group
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evg = nullptr;
-  int groups = 2;  // slot groups (streams); PSAMD_GROUPS=1 runs everything on one stream
+  hipStream_t stream = nullptr;                   // = streams[0]
+  hipStream_t streams[kMaxGroups] = {};           // one per slot group
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evg[kMaxGroups] = {};
+  int groups = 3;  // slot groups (streams); PSAMD_GROUPS=1..4 overrides
   std::string err;
   int precision = PS_PREC_FP64;
   bool have_problem = false;
@@ -1765,25 +1767,27 @@ int run_batch(ps_engine *e, DevWork W) {
   // The slots are split into groups, each driven on its own stream: the
   // kernels of different groups are independent (they share only the item
   // queue and the done counters, both atomic), so the latency-bound
-  // per-slot kernels of one group overlap the model kernel of the other.
-  const int ng = (e->groups > 1 && nslots >= 2 * kWaves) ? 2 : 1;
-  DevWork Wg[2] = {W, W};
-  hipStream_t sg[2] = {e->stream, e->stream2};
-  Wg[0].nslots = ng == 2 ? (nslots + 1) / 2 : nslots;
-  if (ng == 2) {
-    Wg[1].nslots = nslots - Wg[0].nslots;
-    Wg[1].st = e->st + Wg[0].nslots;
-    Wg[1].mem = e->mem + (size_t)Wg[0].nslots * e->slot_doubles;
-    HIP_TRY(hipEventRecord(e->evg, e->stream));
-    HIP_TRY(hipStreamWaitEvent(e->stream2, e->evg, 0));
+  // per-slot kernels of one group overlap the model kernels of the others.
+  int ng = e->groups;
+  while (ng > 1 && nslots < ng * 2 * kWaves) ng--;
+  DevWork Wg[kMaxGroups];
+  hipStream_t *sg = e->streams;
+  HIP_TRY(hipEventRecord(e->evg[0], sg[0]));
+  for (int g = 0, base = 0; g < ng; g++) {
+    Wg[g] = W;
+    Wg[g].nslots = nslots / ng + (g < nslots % ng ? 1 : 0);
+    Wg[g].st = e->st + base;
+    Wg[g].mem = e->mem + (size_t)base * e->slot_doubles;
+    base += Wg[g].nslots;
+    if (g > 0) HIP_TRY(hipStreamWaitEvent(sg[g], e->evg[0], 0));
   }
   const int per_fit = (P.n_iter_max > P.n_iter_min ? P.n_iter_max : P.n_iter_min) + 3;
   const long long rounds = (W.n_items + nslots - 1) / nslots + 1;
   const long long guard = rounds * per_fit * 2 + 64;
   const int poll = 8;
-  if (e->profile && e->pev.size() < (size_t)(4 * poll)) {
+  if (e->profile && e->pev.size() < (size_t)(2 * kMaxGroups * poll)) {
     for (auto ev : e->pev) (void)hipEventDestroy(ev);
-    e->pev.assign(4 * poll, nullptr);
+    e->pev.assign(2 * kMaxGroups * poll, nullptr);
     for (auto &ev : e->pev) HIP_TRY(hipEventCreate(&ev));
   }
   HIP_TRY(hipEventRecord(e->ev0, e->stream));
@@ -1796,18 +1800,18 @@ int run_batch(ps_engine *e, DevWork W) {
         hipLaunchKernelGGL(step_kernel, dim3(ns), dim3(kStepThreads), e->lds_step, sg[g], P, Wk);
         hipLaunchKernelGGL(curv_kernel, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g],
                            P, Wk);
-        if (e->profile) HIP_TRY(hipEventRecord(e->pev[4 * k + 2 * g], sg[g]));
+        if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g)], sg[g]));
         hipLaunchKernelGGL(e->model, dim3(ns * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads), e->lds_model,
                            sg[g], P, Wk);
-        if (e->profile) HIP_TRY(hipEventRecord(e->pev[4 * k + 2 * g + 1], sg[g]));
+        if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g) + 1], sg[g]));
         hipLaunchKernelGGL(e->neq, dim3(ns), dim3(kThreads), e->lds_neq, sg[g], P, Wk);
       }
     }
     HIP_TRY(hipGetLastError());
     it += poll;
-    if (ng == 2) {
-      HIP_TRY(hipEventRecord(e->evg, e->stream2));
-      HIP_TRY(hipStreamWaitEvent(e->stream, e->evg, 0));
+    for (int g = 1; g < ng; g++) {
+      HIP_TRY(hipEventRecord(e->evg[g], sg[g]));
+      HIP_TRY(hipStreamWaitEvent(sg[0], e->evg[g], 0));
     }
     HIP_TRY(hipMemcpyAsync(e->host_done, W.done, 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1815,7 +1819,7 @@ int run_batch(ps_engine *e, DevWork W) {
       for (int k = 0; k < poll; k++)
         for (int g = 0; g < ng; g++) {
           float ms = 0;
-          HIP_TRY(hipEventElapsedTime(&ms, e->pev[4 * k + 2 * g], e->pev[4 * k + 2 * g + 1]));
+          HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + g)], e->pev[2 * (kMaxGroups * k + g) + 1]));
           e->model_ms += ms;
         }
       e->model_launches += poll * ng;
@@ -1858,10 +1862,12 @@ int ps_create(int device, ps_engine **out) {
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PS_ERR_HIP;
   auto *e = new ps_engine();
   e->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&e->evg, hipEventDisableTiming) != hipSuccess ||
+  bool ok = hipSetDevice(device) == hipSuccess;
+  for (int g = 0; g < kMaxGroups; g++)
+    ok = ok && hipStreamCreateWithFlags(&e->streams[g], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&e->evg[g], hipEventDisableTiming) == hipSuccess;
+  e->stream = e->streams[0];
+  if (!ok || hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
       hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     delete e;
@@ -1873,7 +1879,10 @@ int ps_create(int device, ps_engine **out) {
     return PS_ERR_HIP;
   }
   e->cu_count = prop.multiProcessorCount;
-  if (const char *g = getenv("PSAMD_GROUPS")) e->groups = atoi(g) > 1 ? 2 : 1;
+  if (const char *g = getenv("PSAMD_GROUPS")) {
+    const int v = atoi(g);
+    e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);
+  }
   *out = e;
   return PS_OK;
 }
@@ -1881,8 +1890,8 @@ int ps_create(int device, ps_engine **out) {
 int ps_destroy(ps_engine *e) {
   if (!e) return PS_ERR_ARG;
   (void)hipSetDevice(e->device);
-  (void)hipStreamSynchronize(e->stream);
-  if (e->stream2) (void)hipStreamSynchronize(e->stream2);
+  for (int g = 0; g < kMaxGroups; g++)
+    if (e->streams[g]) (void)hipStreamSynchronize(e->streams[g]);
   free_problem(e);
   void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows, e->d_feat, e->d_cgo,
                   e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
@@ -1892,9 +1901,10 @@ int ps_destroy(ps_engine *e) {
   for (auto ev : e->pev) (void)hipEventDestroy(ev);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
-  if (e->evg) (void)hipEventDestroy(e->evg);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
-  if (e->stream2) (void)hipStreamDestroy(e->stream2);
+  for (int g = 0; g < kMaxGroups; g++) {
+    if (e->evg[g]) (void)hipEventDestroy(e->evg[g]);
+    if (e->streams[g]) (void)hipStreamDestroy(e->streams[g]);
+  }
   delete e;
   return PS_OK;
 }
@@ -2132,7 +2142,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   e->lds_step = step_lds(mfit, F);
   e->lds_model = mixed ? mixed_layout(F, nb, mode).total : (e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
   e->lds_neq = neq_lds(mfit, mixed);
-  const size_t lds_max = 160 * 1024;
+  const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
   if (e->lds_step > lds_max || e->lds_neq > lds_max || e->lds_model > lds_max) {
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;

@@ -47,13 +47,15 @@ def test_driver_matches_oracle_and_resumes(short_scan):
     # amplify that on a few (period, pole) trajectories (at lmax 3, per-fit
     # chi2 differences of up to 4e-3 on 2-3 of 60 fits, the rest below 1e-8),
     # which can move a row's best pole to another minimum.  So: period within
-    # 1e-6 relative in every row (one scan step is 2.5e-5 here); rms and chi2
-    # within 1e-4 relative in all but at most one row, and within 1e-2 there.
+    # one scan step (2.5e-5 relative here) in every row and within 1e-6 in all
+    # but at most one; rms and chi2 within 1e-4 relative in all but at most
+    # one row, and within 1e-2 there.
     close = 0
     for g, o in zip(g_rows, o_rows):
-        assert abs(g[0] - o[0]) <= 1e-6 * o[0]        # period
+        assert abs(g[0] - o[0]) <= 2.5e-5 * o[0]       # period
         assert abs(g[1] - o[1]) <= 1e-2 * o[1] and abs(g[2] - o[2]) <= 1e-2 * o[2]
-        close += abs(g[1] - o[1]) <= 1e-4 * o[1] + 1e-6 and abs(g[2] - o[2]) <= 1e-4 * o[2] + 1e-6
+        close += (abs(g[0] - o[0]) <= 1e-6 * o[0] and abs(g[1] - o[1]) <= 1e-4 * o[1] + 1e-6
+                  and abs(g[2] - o[2]) <= 1e-4 * o[2] + 1e-6)
     assert close >= len(g_rows) - 1
     state = open(os.path.join(wd, "period_search_state")).read().split()
     assert int(state[0]) == len(g_rows) and int(state[1]) == 1
