@@ -14,10 +14,10 @@ any implementation).
 
 Forms: --prec fp64 (plain), fp32 / bf16 / bf16x3 (mixed precision), and
 --form learned (mixed precision with the learned proposal: every trial period
-of the step fitted once from the network's start, the best --candidates
-periods and two neighbours each refitted with the plain ten poles; the value
-is then the plain-equivalent rate, 10 fits per trial period covered, and
-lm_fits_per_s the LM fits actually run).
+of the step fitted once from the network's start, in each 1024-period block
+the best --candidates periods and two neighbours each refitted with the plain
+ten poles; the value is then the plain-equivalent rate, 10 fits per trial
+period covered, and lm_fits_per_s the LM fits actually run).
 
 Prints one JSON line with fits/s, the roofline position of the dominant
 kernel (the model kernel: facet stage + brightness-derivative contraction,
@@ -79,14 +79,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--periods", type=int, default=1024, help="trial periods per step")
+    ap.add_argument("--periods", type=int, default=4096, help="trial periods per step")
     ap.add_argument("--slots", type=int, default=0, help="resident fits (0 = engine default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3"), default="bf16x3")
     ap.add_argument("--form", choices=("plain", "learned"), default="learned")
-    ap.add_argument("--candidates", type=int, default=4, help="learned form: periods verified per step")
+    ap.add_argument("--candidates", type=int, default=4, help="learned form: periods verified per block")
+    ap.add_argument("--block", type=int, default=1024, help="learned form: trial periods per block")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -132,7 +133,7 @@ def main() -> int:
         nonlocal n_next, lm_fits
         if args.form == "learned":
             rows, _, nf, _ = scan_learned(eng, inp, prop, conw_r, n_next, n_next + P - 1,
-                                          n_candidates=args.candidates, halo=2, mirror=False)
+                                          n_candidates=args.candidates, halo=2, block=args.block)
         else:
             rows = eng.scan(inp.freq_start, inp.freq_step, n_next, n_next + P - 1, conw_r)
             nf = 10 * P
@@ -191,9 +192,10 @@ def main() -> int:
         "lm_fits_per_s": round(lm_fits / wall, 2),
     }
     if args.form == "learned":
-        out["config"]["learned"] = (f"1 proposed start per trial period; {args.candidates} best periods per step "
-                                    f"and 2 neighbours each refitted from the 10 plain poles; value = 10 x trial "
-                                    f"periods covered per second")
+        out["config"]["learned"] = (f"1 proposed start per trial period; in each {args.block}-period block the "
+                                    f"{args.candidates} best periods and 2 neighbours each refitted from the 10 "
+                                    f"plain poles (in the next block's batch); value = 10 x trial periods covered "
+                                    f"per second")
     if not args.no_cpu:
         try:
             ncpu = min(len(os.sched_getaffinity(0)), 16)

@@ -1633,8 +1633,9 @@ struct ps_engine {
   int device = 0;
   hipStream_t stream = nullptr;                   // = streams[0]
   hipStream_t streams[kMaxGroups] = {};           // one per slot group
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evg[kMaxGroups] = {};
-  int groups = 3;  // slot groups (streams); PSAMD_GROUPS=1..4 overrides
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, evg[kMaxGroups] = {};
+  bool use_graph = true;  // replay each batch's poll from a HIP graph (PSAMD_GRAPH=0: plain launches)
+  int groups = 1;  // slot groups (streams); PSAMD_GROUPS=1..4 overrides
   std::string err;
   int precision = PS_PREC_FP64;
   bool have_problem = false;
@@ -1790,9 +1791,11 @@ int run_batch(ps_engine *e, DevWork W) {
     e->pev.assign(2 * kMaxGroups * poll, nullptr);
     for (auto &ev : e->pev) HIP_TRY(hipEventCreate(&ev));
   }
-  HIP_TRY(hipEventRecord(e->ev0, e->stream));
-  long long it = 0;
-  while (true) {
+  // One poll = `poll` LM iterations of every group, then the groups join
+  // stream 0.  Without profiling events the poll is captured once per batch
+  // into a HIP graph (its kernel arguments do not change between polls) and
+  // replayed; profiling records events between the launches instead.
+  auto enqueue_poll = [&](bool prof) -> int {
     for (int k = 0; k < poll; k++) {
       for (int g = 0; g < ng; g++) {
         const DevWork &Wk = Wg[g];
@@ -1800,19 +1803,50 @@ int run_batch(ps_engine *e, DevWork W) {
         hipLaunchKernelGGL(step_kernel, dim3(ns), dim3(kStepThreads), e->lds_step, sg[g], P, Wk);
         hipLaunchKernelGGL(curv_kernel, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g],
                            P, Wk);
-        if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g)], sg[g]));
+        if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g)], sg[g]));
         hipLaunchKernelGGL(e->model, dim3(ns * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads), e->lds_model,
                            sg[g], P, Wk);
-        if (e->profile) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g) + 1], sg[g]));
+        if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g) + 1], sg[g]));
         hipLaunchKernelGGL(e->neq, dim3(ns), dim3(kThreads), e->lds_neq, sg[g], P, Wk);
       }
     }
     HIP_TRY(hipGetLastError());
-    it += poll;
     for (int g = 1; g < ng; g++) {
       HIP_TRY(hipEventRecord(e->evg[g], sg[g]));
       HIP_TRY(hipStreamWaitEvent(sg[0], e->evg[g], 0));
     }
+    return PS_OK;
+  };
+  hipGraphExec_t gexec = nullptr;
+  if (e->use_graph && !e->profile) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(sg[0], hipStreamCaptureModeThreadLocal));
+    HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other groups' streams into the capture
+    for (int g = 1; g < ng; g++) HIP_TRY(hipStreamWaitEvent(sg[g], e->evf, 0));
+    int rc_cap = enqueue_poll(false);
+    hipError_t err_end = hipStreamEndCapture(sg[0], &graph);
+    if (rc_cap) return rc_cap;
+    HIP_TRY(err_end);
+    hipError_t err_inst = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(err_inst);
+  }
+  struct GraphGuard {
+    hipGraphExec_t &g;
+    ~GraphGuard() {
+      if (g) (void)hipGraphExecDestroy(g);
+    }
+  } guard_graph{gexec};
+  HIP_TRY(hipEventRecord(e->ev0, e->stream));
+  long long it = 0;
+  while (true) {
+    if (gexec) {
+      HIP_TRY(hipGraphLaunch(gexec, sg[0]));
+    } else {
+      const int rc_p = enqueue_poll(e->profile);
+      if (rc_p) return rc_p;
+    }
+    it += poll;
     HIP_TRY(hipMemcpyAsync(e->host_done, W.done, 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->profile) {
@@ -1868,6 +1902,7 @@ int ps_create(int device, ps_engine **out) {
          hipEventCreateWithFlags(&e->evg[g], hipEventDisableTiming) == hipSuccess;
   e->stream = e->streams[0];
   if (!ok || hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&e->evf, hipEventDisableTiming) != hipSuccess ||
       hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     delete e;
@@ -1879,6 +1914,7 @@ int ps_create(int device, ps_engine **out) {
     return PS_ERR_HIP;
   }
   e->cu_count = prop.multiProcessorCount;
+  if (const char *gr = getenv("PSAMD_GRAPH")) e->use_graph = atoi(gr) != 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
     e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);
@@ -1901,6 +1937,7 @@ int ps_destroy(ps_engine *e) {
   for (auto ev : e->pev) (void)hipEventDestroy(ev);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->evf) (void)hipEventDestroy(e->evf);
   for (int g = 0; g < kMaxGroups; g++) {
     if (e->evg[g]) (void)hipEventDestroy(e->evg[g]);
     if (e->streams[g]) (void)hipStreamDestroy(e->streams[g]);

@@ -169,41 +169,66 @@ def _rows_from_fits(fits: np.ndarray, per: int, n_points: int) -> np.ndarray:
     return rows
 
 
-def scan_learned(eng: Engine, inp: Input, prop: Proposal, conw_r: float, n_from: int, n_to: int,
-                 n_candidates: int = 8, halo: int = 2, mirror: bool = True):
-    """Rows n_from..n_to: every trial period fitted from the proposed start
-    (and its mirror pole); the n_candidates periods with the lowest rms, and
-    halo periods on either side of each, refitted with the reference's ten
-    poles (Engine.scan).  Returns (rows, verified mask, LM fits run, rows of
-    the proposed starts alone)."""
-    n = np.arange(n_from, n_to + 1)
-    freq = inp.freq_start - (n - 1) * inp.freq_step
-    feats = eng.lc_features(freq, NBINS)
-    starts = prop.starts(inp, freq, feats)
-    per = 2 if mirror else 1
-    if not mirror:
-        starts = starts[0::2]
-    fits = eng.fit_batch_from(starts, conw_r)
-    learned = _rows_from_fits(fits, per, inp.ndata)
-    rows = learned.copy()
-    verified = np.zeros(len(n), bool)
-    order = np.argsort(rows["rms"], kind="stable")
+def _pick_candidates(rms: np.ndarray, n_candidates: int, halo: int) -> np.ndarray:
+    """Mask of the n_candidates lowest-rms rows (no two within halo) and
+    halo rows on either side of each."""
+    verified = np.zeros(len(rms), bool)
     picked = []
-    for k in order:
+    for k in np.argsort(rms, kind="stable"):
         if len(picked) >= n_candidates:
             break
         if any(abs(int(k) - q) <= halo for q in picked):
             continue
         picked.append(int(k))
     for k in picked:
-        verified[max(0, k - halo):min(len(n), k + halo + 1)] = True
-    # the verified periods in one batch of plain fits (the reference's poles),
-    # reduced to rows as Engine.scan does
-    vk = np.flatnonzero(verified)
+        verified[max(0, k - halo):min(len(rms), k + halo + 1)] = True
+    return verified
+
+
+def scan_learned(eng: Engine, inp: Input, prop: Proposal, conw_r: float, n_from: int, n_to: int,
+                 n_candidates: int = 4, halo: int = 2, mirror: bool = False, block: int = 1024):
+    """Rows n_from..n_to, in blocks of `block` trial periods: every period
+    fitted from the proposed start (and, if asked, its mirror pole); in each
+    block the n_candidates periods with the lowest rms and halo periods on
+    either side of each are refitted from the reference's ten poles.  Those
+    refits run in the same batch as the next block's proposed fits (one
+    batch of LM fits per block instead of two).  Returns (rows, verified
+    mask, number of LM fits run, rows of the proposed starts alone); only
+    verified rows are judged results."""
+    n = np.arange(n_from, n_to + 1)
+    freq = inp.freq_start - (n - 1) * inp.freq_step
+    per = 2 if mirror else 1
     npol = inp.problem.n_poles
-    plain = eng.fit_batch(np.repeat(freq[vk], npol), np.tile(np.arange(npol), len(vk)), conw_r)
-    rows[vk] = _rows_from_fits(plain, npol, inp.ndata)
-    return rows, verified, len(fits) + len(plain), learned
+    learned = np.zeros(len(n), dtype=ROW_DTYPE)
+    rows = np.zeros(len(n), dtype=ROW_DTYPE)
+    verified = np.zeros(len(n), bool)
+    n_fits = 0
+    pending = np.zeros(0, int)  # positions of the previous block awaiting their plain refits
+    blocks = list(range(0, len(n), block))
+    for bi in range(len(blocks) + 1):
+        parts = []
+        if bi < len(blocks):
+            b0, b1 = blocks[bi], min(len(n), blocks[bi] + block)
+            starts = prop.starts(inp, freq[b0:b1], eng.lc_features(freq[b0:b1], NBINS))
+            parts.append(starts if mirror else starts[0::2])
+        if len(pending):
+            parts.append(reference_starts(inp, freq[pending]))
+        if not parts:
+            break
+        fits = eng.fit_batch_from(np.concatenate(parts), conw_r)
+        n_fits += len(fits)
+        m = 0
+        new_pending = np.zeros(0, int)
+        if bi < len(blocks):
+            m = (b1 - b0) * per
+            learned[b0:b1] = _rows_from_fits(fits[:m], per, inp.ndata)
+            rows[b0:b1] = learned[b0:b1]
+            new_pending = b0 + np.flatnonzero(_pick_candidates(learned["rms"][b0:b1], n_candidates, halo))
+        if len(pending):
+            rows[pending] = _rows_from_fits(fits[m:], npol, inp.ndata)
+            verified[pending] = True
+        pending = new_pending
+    return rows, verified, n_fits, learned
 
 
 # ---------------------------------------------------------------- training

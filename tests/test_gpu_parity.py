@@ -224,3 +224,28 @@ def test_mixed_chi2_minimum_matches_fp64(tmp_path, name, prec, tol_chi2):
     orc.prepare(inp.conw_r0)
     o = orc.period(k64 + 1)
     assert _rel(c64, o[2]) < 1e-6 and _rel(p64, o[0]) < 1e-9
+
+
+@pytest.mark.parametrize("groups", ["1", "3"])
+def test_graph_replay_matches_plain_launches(small_incl_input, monkeypatch, groups):
+    """The poll captured as a HIP graph and replayed (the default) gives the
+    same fits, bit for bit, as launching its kernels one by one; with one
+    slot group and with three groups on their own streams (fork/join inside
+    the graph).  A batch replays the graph about 7 times, with a copy and a
+    synchronisation between replays."""
+    from periodsearch_amd import Engine, Input
+
+    inp = Input(small_incl_input)
+    freqs = [inp.freq_start - n * inp.freq_step for n in (0, 7, 13) for _ in range(10)]
+    poles = [m for _ in range(3) for m in range(10)]
+    out = {}
+    for graph in ("1", "0"):
+        monkeypatch.setenv("PSAMD_GRAPH", graph)
+        monkeypatch.setenv("PSAMD_GROUPS", groups)
+        eng = Engine(0)
+        eng.set_problem(inp.problem, keep=inp)
+        out[graph] = [eng.fit_batch(freqs, poles, inp.conw_r0) for _ in range(2)]
+        eng.close()
+    for a, b in zip(out["1"], out["0"]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out["1"][0], out["1"][1])
