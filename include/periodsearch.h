@@ -132,6 +132,21 @@ int ps_set_precision(ps_engine *e, int precision);
 
 int ps_set_problem(ps_engine *e, const ps_problem *p);
 
+/* Several asteroids at once (no counterpart in the reference, which scans
+ * one asteroid per process): the problems may differ in points, light
+ * curves, lmax/mmax, facets and stop rule; their fits share the engine's
+ * slots and work queue (ps_fit_batch_multi).  With more than one problem set,
+ * the one-problem calls (ps_precalc, ps_scan, ps_fit_batch, ps_eval_mrqcof,
+ * ps_lc_features, ps_fit_batch_from) return PS_ERR_STATE. */
+int ps_set_problems(ps_engine *e, const ps_problem *p, int n);
+
+/* n fits over the problems of ps_set_problems: item i fits problem prob[i]
+ * at trial frequency freq[i] (1/h) from that problem's pole pole[i];
+ * conw_r[k] is problem k's convexity weight.  Items run in any order through
+ * the slots; out[i] is item i's result. */
+int ps_fit_batch_multi(ps_engine *e, const int *prob, const double *freq, const int *pole, int n,
+                       const double *conw_r, ps_fit_out *out);
+
 /* Upper bound on concurrently resident fits (slots).  Default 2048.  Each
  * slot holds one fit's LM state and Jacobian in HBM. */
 int ps_set_slots(ps_engine *e, int max_slots);

@@ -82,6 +82,8 @@ def _load() -> C.CDLL:
         "ps_set_precision": (C.c_int, [vp, C.c_int]),
         "ps_set_slots": (C.c_int, [vp, C.c_int]),
         "ps_set_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
+        "ps_set_problems": (C.c_int, [vp, C.POINTER(PsProblem), C.c_int]),
+        "ps_fit_batch_multi": (C.c_int, [vp, _ip, _dp, _ip, C.c_int, _dp, vp]),
         "ps_precalc": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, _dp, _ip]),
         "ps_scan": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, vp]),
         "ps_fit_batch": (C.c_int, [vp, _dp, _ip, C.c_int, C.c_double, vp]),
@@ -120,6 +122,7 @@ lib = _load()
 
 # Symbols declared in include/*.h (checked by the CPU test suite).
 EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_set_slots", "ps_set_problem",
+            "ps_set_problems", "ps_fit_batch_multi",
             "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
             "ps_input_scalars", "ps_format_row", "ps_shape_basis",
@@ -229,6 +232,24 @@ class Engine:
     def set_problem(self, problem: PsProblem, keep=None):
         self._keep = keep  # keep the arrays behind the pointers alive
         self._check(lib.ps_set_problem(self._h, C.byref(problem)), "ps_set_problem")
+
+    def set_problems(self, problems, keep=None):
+        """Several asteroids at once (a list of PsProblem)."""
+        arr = (PsProblem * len(problems))(*problems)
+        self._keep = (keep, arr)
+        self._check(lib.ps_set_problems(self._h, arr, len(problems)), "ps_set_problems")
+
+    def fit_batch_multi(self, prob, freq, pole, conw_r) -> np.ndarray:
+        """Fits over the problems of set_problems: item i fits problem prob[i]
+        at freq[i] from pole[i]; conw_r[k] is problem k's convexity weight."""
+        prob = np.ascontiguousarray(prob, dtype=np.int32)
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        pole = np.ascontiguousarray(pole, dtype=np.int32)
+        conw = np.ascontiguousarray(conw_r, dtype=np.float64)
+        out = np.zeros(len(freq), dtype=FIT_DTYPE)
+        self._check(lib.ps_fit_batch_multi(self._h, _i(prob), _d(freq), _i(pole), len(freq), _d(conw),
+                                           out.ctypes.data), "ps_fit_batch_multi")
+        return out
 
     def dims(self):
         nc, ma, mf = C.c_int(), C.c_int(), C.c_int()

@@ -107,7 +107,7 @@ struct DevProblem {
 // block of slot memory (SlotMem).
 struct SlotState {
   int item, phase, niter, err;
-  int eval, sel, pad0, pad1;
+  int eval, sel, prob, pad1;  // prob: index of the fit's problem (asteroid) in the batch
   double lam, ochisq, chisq, chitr, dev_new, dev_old, iter_diff, rchisq;
   double chck[3], chtr[3];
 };
@@ -146,6 +146,8 @@ struct DevWork {
   double conw_r;
   ps_fit_out *out;          // [n_items] (modes 0, 2)
   double *cg_out;           // [n_items][ma] final parameters, or null
+  const int *item_prob;     // [n_items] problem of each item (mode 0, several problems), or null: problem 0
+  const double *conw_rs;    // [n_problems] convexity weight per problem, or null: conw_r
   double *ev_alpha, *ev_beta, *ev_chi, *ev_chck;  // (mode 1)
   int *queue, *done;   // done[1] counts model evaluations
   SlotState *st;
@@ -792,6 +794,17 @@ __device__ int gauss_solve(double *buf, int n, double *x) {
   return 0;
 }
 
+// Problem of a slot.  One problem (MULTI false): the kernel argument P0, as
+// constant as any other argument.  Several (MULTI true): the slot's entry of
+// the device array of problems.
+template <bool MULTI>
+__device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, const DevProblem *probs, int idx) {
+  if constexpr (MULTI)
+    return probs[__builtin_amdgcn_readfirstlane(idx)];
+  else
+    return P0;
+}
+
 // step_kernel: one workgroup per slot.  Finishes a fit whose loop predicate
 // is false (per-fit summary) and takes the next fit from the queue, or damps
 // the accepted normal equations and solves them, forming the trial
@@ -799,26 +812,27 @@ __device__ int gauss_solve(double *buf, int n, double *x) {
 // LDS: the solver's two augmented matrices and pivot exchange; the final
 // curv of a finished fit (area/g [F]) reuses the same space.
 constexpr int kStepThreads = kThreads;
-__global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWork W) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kStepThreads, 4) step_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+                                                          DevWork W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
-  const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
   SlotState &S = W.st[slot];
-  const SlotMem M = slot_mem(ma, mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
-  double *A = sm, *area = sm, *g = area + P.nfac;
-  __shared__ int s_phase, s_go, s_item;
+  __shared__ int s_phase, s_go, s_item, s_prob;
 
   if (tid == 0) {
     int ph = S.phase;
     int go = 0;
     if (ph == PH_ITER) {
       // loop predicate (period_search_BOINC.cpp:868)
-      go = (((S.niter < P.n_iter_max) && (S.iter_diff > P.iter_diff_max)) || (S.niter < P.n_iter_min)) ? 1 : 0;
+      const DevProblem &Pc = pick_problem<MULTI>(P0, probs, S.prob);
+      go = (((S.niter < Pc.n_iter_max) && (S.iter_diff > Pc.iter_diff_max)) || (S.niter < Pc.n_iter_min)) ? 1 : 0;
     }
     s_phase = ph;
     s_go = go;
     s_item = S.item;
+    s_prob = S.prob;
   }
   __syncthreads();
   int ph = s_phase;
@@ -829,6 +843,10 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
   if (ph == PH_ITER && !s_go) {
     // per-fit summary (period_search_BOINC.cpp:900-911); totarea from the
     // shape of the last curv() call.
+    const DevProblem &P = pick_problem<MULTI>(P0, probs, s_prob);
+    const int ma = P.ma, nc = P.ncoef;
+    const SlotMem M = slot_mem(ma, P.mfit, P.npad);
+    double *area = sm, *g = area + P.nfac;
     eval_curv(P, mem + M.alast, area, g);
     if (W.cg_out)
       for (int l = tid; l < ma; l += kStepThreads) W.cg_out[(size_t)s_item * ma + l] = mem[M.a + l];
@@ -861,6 +879,7 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
       int item = atomicAdd(W.queue, 1);
       s_item = item;
       s_phase = item < W.n_items ? PH_INIT : PH_DRAINED;
+      s_prob = (item < W.n_items && W.item_prob) ? W.item_prob[item] : 0;
     }
     __syncthreads();
     const int item = s_item;
@@ -871,6 +890,9 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
       }
       return;
     }
+    const DevProblem &P = pick_problem<MULTI>(P0, probs, s_prob);
+    const int ma = P.ma;
+    const SlotMem M = slot_mem(ma, P.mfit, P.npad);
     double *a = mem + M.a, *atry = mem + M.atry;
     if (W.mode != 0) {  // given starting parameters (evaluation or fit)
       for (int l = tid; l < ma; l += kStepThreads) a[l] = W.cg_in[(size_t)item * ma + l];
@@ -880,6 +902,7 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
     for (int l = tid; l < ma; l += kStepThreads) atry[l] = a[l];
     if (tid == 0) {
       S.item = item;
+      S.prob = s_prob;
       S.phase = PH_INIT;
       S.eval = 1;
       S.err = 0;
@@ -887,6 +910,10 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
     return;
   }
   // damped normal equations and solve (mrqmin.c:53-79)
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, s_prob);
+  const int mfit = P.mfit, ma = P.ma;
+  const SlotMem M = slot_mem(ma, mfit, P.npad);
+  double *A = sm;
   const double *aacc = mem + (S.sel ? M.alpha1 : M.alpha0);
   const double *bacc = mem + (S.sel ? M.beta1 : M.beta0);
   double *da = mem + M.da;
@@ -922,10 +949,12 @@ __global__ void __launch_bounds__(kStepThreads) step_kernel(DevProblem P, DevWor
 // one thread per facet (period_search/curv.c:10-38, reference operation
 // order); Area and g go to the slot for the model and normal-equation
 // kernels.  blockIdx.x = chunk * nslots + slot.
-__global__ void __launch_bounds__(kThreads) curv_kernel(DevProblem P, DevWork W) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kThreads) curv_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
 #pragma clang fp contract(off)
   const int slot = blockIdx.x % W.nslots, chunk = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
   const int f = chunk * kThreads + threadIdx.x;
   if (f >= P.nfac) return;
   const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
@@ -939,11 +968,15 @@ __global__ void __launch_bounds__(kThreads) curv_kernel(DevProblem P, DevWork W)
 #ifndef PS_MODEL_MIN_WAVES
 #define PS_MODEL_MIN_WAVES 1
 #endif
-template <int NCB>
-__global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(DevProblem P, DevWork W) {
+template <int NCB, bool MULTI>
+__global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(DevProblem P0,
+                                                                            const DevProblem *__restrict__ probs,
+                                                                            DevWork W) {
   extern __shared__ double sm[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
+  if (tile >= P.ntiles) return;
   const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
   double *x = sm, *blm = x + kMaxMa, *area = blm + 32, *g = area + P.nfac;
@@ -971,11 +1004,14 @@ __global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(Dev
 // LDS: x[kMaxMa] blm[32] area[F] geom[kTilePts] W[kFacetChunk][kTilePts]
 //      Dsph chunk[kFacetChunk][16 NB]; the last two regions are reused for
 //      the partial-sum reduction at the end.
-template <int NB>
-__global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P, DevWork W) {
+template <int NB, bool MULTI>
+__global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+                                                              DevWork W) {
   extern __shared__ double sm[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
+  if (tile >= P.ntiles64) return;
   constexpr int NC = 16 * NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
@@ -1161,11 +1197,14 @@ __device__ inline unsigned bf16_bits(float v) {  // round to nearest even
   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
 }
 
-template <int NB, int MODE>
-__global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P, DevWork W) {
+template <int NB, int MODE, bool MULTI>
+__global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+                                                               DevWork W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
+  if (tile >= P.ntiles64) return;
   constexpr bool BF = MODE != 0, SPLIT = MODE == 2;
   constexpr int NC = 16 * NB;
   constexpr int WS = kFacetChunk + (BF ? 8 : 4);   // padded row, elements (80 / 144 bytes)
@@ -1348,13 +1387,14 @@ __global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P, Dev
 // the trial parameters, then accept/reject and the driver bookkeeping.
 // LDS: staged rows (neq_jn_doubles), aux [3*kNeqChunk], dave [ma+1], x [ma],
 // scalars [4], dave/ave [ma].
-template <int NT, bool MIX>
-__global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) {
+template <int NT, bool MIX, bool MULTI>
+__global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
   SlotState &S = W.st[slot];
   const int ph = S.phase;
   if (ph != PH_INIT && ph != PH_ITER) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, S.prob);
   const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
   const SlotMem M = slot_mem(ma, mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
@@ -1370,11 +1410,12 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P, DevWork W) 
     conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1);
     __threadfence_block();
     __syncthreads();
-    normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_r, wst, dave, drat, btr, scal, atr);
+    normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, btr,
+                                   scal, atr);
     __threadfence_block();
     __syncthreads();
   }
-  const double conw_r = W.conw_r;
+  const double conw_r = W.conw_rs ? W.conw_rs[S.prob] : W.conw_r;
   double *a = mem + M.a, *atry = mem + M.atry, *alast = mem + M.alast;
   if (ph == PH_INIT) {
     if (W.mode == 1) {
@@ -1545,55 +1586,57 @@ __global__ void make_items(double freq_start, double freq_step, int n_from, int 
   pole[i] = i % n_poles;
 }
 
-using ModelFn = void (*)(DevProblem, DevWork);
+using ModelFn = void (*)(DevProblem, const DevProblem *, DevWork);
 
+template <bool MULTI>
 ModelFn pick_model(int ncp) {
   switch (ncp) {
-    case 8: return model_kernel<8>;
-    case 16: return model_kernel<16>;
-    case 24: return model_kernel<24>;
-    case 32: return model_kernel<32>;
-    case 40: return model_kernel<40>;
-    case 48: return model_kernel<48>;
-    case 56: return model_kernel<56>;
-    case 64: return model_kernel<64>;
-    case 72: return model_kernel<72>;
-    case 80: return model_kernel<80>;
-    case 88: return model_kernel<88>;
-    case 96: return model_kernel<96>;
-    case 104: return model_kernel<104>;
-    case 112: return model_kernel<112>;
-    case 120: return model_kernel<120>;
-    case 128: return model_kernel<128>;
+    case 8: return model_kernel<8, MULTI>;
+    case 16: return model_kernel<16, MULTI>;
+    case 24: return model_kernel<24, MULTI>;
+    case 32: return model_kernel<32, MULTI>;
+    case 40: return model_kernel<40, MULTI>;
+    case 48: return model_kernel<48, MULTI>;
+    case 56: return model_kernel<56, MULTI>;
+    case 64: return model_kernel<64, MULTI>;
+    case 72: return model_kernel<72, MULTI>;
+    case 80: return model_kernel<80, MULTI>;
+    case 88: return model_kernel<88, MULTI>;
+    case 96: return model_kernel<96, MULTI>;
+    case 104: return model_kernel<104, MULTI>;
+    case 112: return model_kernel<112, MULTI>;
+    case 120: return model_kernel<120, MULTI>;
+    case 128: return model_kernel<128, MULTI>;
     default: return nullptr;
   }
 }
 
+template <bool MULTI>
 ModelFn pick_mfma(int ncoef) {
   switch ((ncoef + 15) / 16) {
-    case 1: return model_mfma_kernel<1>;
-    case 2: return model_mfma_kernel<2>;
-    case 3: return model_mfma_kernel<3>;
-    case 4: return model_mfma_kernel<4>;
-    case 5: return model_mfma_kernel<5>;
-    case 6: return model_mfma_kernel<6>;
-    case 7: return model_mfma_kernel<7>;
-    case 8: return model_mfma_kernel<8>;
+    case 1: return model_mfma_kernel<1, MULTI>;
+    case 2: return model_mfma_kernel<2, MULTI>;
+    case 3: return model_mfma_kernel<3, MULTI>;
+    case 4: return model_mfma_kernel<4, MULTI>;
+    case 5: return model_mfma_kernel<5, MULTI>;
+    case 6: return model_mfma_kernel<6, MULTI>;
+    case 7: return model_mfma_kernel<7, MULTI>;
+    case 8: return model_mfma_kernel<8, MULTI>;
     default: return nullptr;
   }
 }
 
-template <int MODE>
+template <int MODE, bool MULTI>
 ModelFn pick_mixed(int ncoef) {
   switch ((ncoef + 15) / 16) {
-    case 1: return model_mixed_kernel<1, MODE>;
-    case 2: return model_mixed_kernel<2, MODE>;
-    case 3: return model_mixed_kernel<3, MODE>;
-    case 4: return model_mixed_kernel<4, MODE>;
-    case 5: return model_mixed_kernel<5, MODE>;
-    case 6: return model_mixed_kernel<6, MODE>;
-    case 7: return model_mixed_kernel<7, MODE>;
-    case 8: return model_mixed_kernel<8, MODE>;
+    case 1: return model_mixed_kernel<1, MODE, MULTI>;
+    case 2: return model_mixed_kernel<2, MODE, MULTI>;
+    case 3: return model_mixed_kernel<3, MODE, MULTI>;
+    case 4: return model_mixed_kernel<4, MODE, MULTI>;
+    case 5: return model_mixed_kernel<5, MODE, MULTI>;
+    case 6: return model_mixed_kernel<6, MODE, MULTI>;
+    case 7: return model_mixed_kernel<7, MODE, MULTI>;
+    case 8: return model_mixed_kernel<8, MODE, MULTI>;
     default: return nullptr;
   }
 }
@@ -1616,14 +1659,14 @@ size_t neq_lds(int mfit, bool mix) {
   return ((size_t)neq_jn_doubles(mfit, mix) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8) * sizeof(double);
 }
 
-using NeqFn = void (*)(DevProblem, DevWork);
-template <bool MIX>
+using NeqFn = void (*)(DevProblem, const DevProblem *, DevWork);
+template <bool MIX, bool MULTI>
 NeqFn pick_neq(int mfit) {
   const int nb = (mfit + 15) / 16, ntile = nb * (nb + 1) / 2, nt = (ntile + 3) / 4;
-  if (nt <= 1) return neq_kernel<1, MIX>;
-  if (nt <= 3) return neq_kernel<3, MIX>;
-  if (nt <= 6) return neq_kernel<6, MIX>;
-  if (nt <= 9) return neq_kernel<9, MIX>;
+  if (nt <= 1) return neq_kernel<1, MIX, MULTI>;
+  if (nt <= 3) return neq_kernel<3, MIX, MULTI>;
+  if (nt <= 6) return neq_kernel<6, MIX, MULTI>;
+  if (nt <= 9) return neq_kernel<9, MIX, MULTI>;
   return nullptr;
 }
 
@@ -1639,7 +1682,14 @@ struct ps_engine {
   std::string err;
   int precision = PS_PREC_FP64;
   bool have_problem = false;
-  DevProblem P{};
+  DevProblem P{};                 // problem 0 (host copy of the device view)
+  DevProblem G{};                 // maxima over the problems of the batch
+  const DevProblem *d_probs = nullptr;  // [n_problems] device views
+  int n_problems = 0;
+  std::vector<int> prob_npoles;   // pole count of each problem
+  int *d_iprob = nullptr;         // [item_cap] problem of each item
+  double *d_conw = nullptr;       // [conw_cap] convexity weight per problem
+  size_t iprob_cap = 0, conw_cap = 0;
   std::vector<void *> prob_allocs;
   int n_poles = 0, cu_count = 0, max_slots = 2048;
   float feat_alpha = 0, feat_aspect = 0;  // mean solar phase angle, mean |Earth latitude| [rad]
@@ -1648,6 +1698,8 @@ struct ps_engine {
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
+  void (*step)(DevProblem, const DevProblem *, DevWork) = nullptr;
+  void (*curv)(DevProblem, const DevProblem *, DevWork) = nullptr;
   bool use_mfma = true;   // FP64: matrix-core contraction (default) or the VALU kernel
   // slot memory
   SlotState *st = nullptr;
@@ -1691,6 +1743,8 @@ void free_problem(ps_engine *e) {
   for (void *p : e->prob_allocs) (void)hipFree(p);
   e->prob_allocs.clear();
   e->have_problem = false;
+  e->d_probs = nullptr;
+  e->n_problems = 0;
 }
 
 int ensure_cg(ps_engine *e, size_t n) {
@@ -1732,7 +1786,7 @@ int ensure_rows(ps_engine *e, size_t n) {
 }
 
 int ensure_slots(ps_engine *e, int nslots) {
-  const SlotMem M = slot_mem(e->P.ma, e->P.mfit, e->P.npad);
+  const SlotMem M = slot_mem(e->G.ma, e->G.mfit, e->G.npad);  // large enough for every problem's layout
   if ((size_t)nslots <= e->mem_slots && M.total == e->slot_doubles) return PS_OK;
   if (e->mem) (void)hipFree(e->mem);
   if (e->st) (void)hipFree(e->st);
@@ -1761,7 +1815,9 @@ int run_batch(ps_engine *e, DevWork W) {
   W.st = e->st;
   W.mem = e->mem;
   W.slot_doubles = e->slot_doubles;
-  const DevProblem &P = e->P;
+  const DevProblem &P = e->G;  // grid sizes and iteration bound: maxima over the problems
+  const DevProblem &P0 = e->P;  // the problem when there is one
+  const DevProblem *probs = e->d_probs;
   hipLaunchKernelGGL(reset_batch, dim3((nslots + 255) / 256), dim3(256), 0, e->stream, e->st, nslots, W.queue,
                      W.done);
   HIP_TRY(hipGetLastError());
@@ -1800,14 +1856,14 @@ int run_batch(ps_engine *e, DevWork W) {
       for (int g = 0; g < ng; g++) {
         const DevWork &Wk = Wg[g];
         const int ns = Wk.nslots;
-        hipLaunchKernelGGL(step_kernel, dim3(ns), dim3(kStepThreads), e->lds_step, sg[g], P, Wk);
-        hipLaunchKernelGGL(curv_kernel, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g],
-                           P, Wk);
+        hipLaunchKernelGGL(e->step, dim3(ns), dim3(kStepThreads), e->lds_step, sg[g], P0, probs, Wk);
+        hipLaunchKernelGGL(e->curv, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g], P0,
+                           probs, Wk);
         if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g)], sg[g]));
         hipLaunchKernelGGL(e->model, dim3(ns * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads), e->lds_model,
-                           sg[g], P, Wk);
+                           sg[g], P0, probs, Wk);
         if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g) + 1], sg[g]));
-        hipLaunchKernelGGL(e->neq, dim3(ns), dim3(kThreads), e->lds_neq, sg[g], P, Wk);
+        hipLaunchKernelGGL(e->neq, dim3(ns), dim3(kThreads), e->lds_neq, sg[g], P0, probs, Wk);
       }
     }
     HIP_TRY(hipGetLastError());
@@ -1875,100 +1931,25 @@ int run_batch(ps_engine *e, DevWork W) {
   return PS_OK;
 }
 
-int check_ready(ps_engine *e) {
+int check_ready(ps_engine *e, bool single = true) {
   if (!e) return PS_ERR_ARG;
   if (!e->have_problem) {
     e->err = "no problem set (ps_set_problem)";
+    return PS_ERR_STATE;
+  }
+  if (single && e->n_problems != 1) {
+    e->err = "this call works on one problem; several are set (ps_set_problems): use ps_fit_batch_multi";
     return PS_ERR_STATE;
   }
   HIP_TRY(hipSetDevice(e->device));
   return PS_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int ps_create(int device, ps_engine **out) {
-  if (!out) return PS_ERR_ARG;
-  *out = nullptr;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PS_ERR_HIP;
-  auto *e = new ps_engine();
-  e->device = device;
-  bool ok = hipSetDevice(device) == hipSuccess;
-  for (int g = 0; g < kMaxGroups; g++)
-    ok = ok && hipStreamCreateWithFlags(&e->streams[g], hipStreamNonBlocking) == hipSuccess &&
-         hipEventCreateWithFlags(&e->evg[g], hipEventDisableTiming) == hipSuccess;
-  e->stream = e->streams[0];
-  if (!ok || hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&e->evf, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-    delete e;
-    return PS_ERR_HIP;
-  }
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
-    delete e;
-    return PS_ERR_HIP;
-  }
-  e->cu_count = prop.multiProcessorCount;
-  if (const char *gr = getenv("PSAMD_GRAPH")) e->use_graph = atoi(gr) != 0;
-  if (const char *g = getenv("PSAMD_GROUPS")) {
-    const int v = atoi(g);
-    e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);
-  }
-  *out = e;
-  return PS_OK;
-}
-
-int ps_destroy(ps_engine *e) {
-  if (!e) return PS_ERR_ARG;
-  (void)hipSetDevice(e->device);
-  for (int g = 0; g < kMaxGroups; g++)
-    if (e->streams[g]) (void)hipStreamSynchronize(e->streams[g]);
-  free_problem(e);
-  void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows, e->d_feat, e->d_cgo,
-                  e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
-  for (void *p : bufs)
-    if (p) (void)hipFree(p);
-  if (e->host_done) (void)hipHostFree(e->host_done);
-  for (auto ev : e->pev) (void)hipEventDestroy(ev);
-  if (e->ev0) (void)hipEventDestroy(e->ev0);
-  if (e->ev1) (void)hipEventDestroy(e->ev1);
-  if (e->evf) (void)hipEventDestroy(e->evf);
-  for (int g = 0; g < kMaxGroups; g++) {
-    if (e->evg[g]) (void)hipEventDestroy(e->evg[g]);
-    if (e->streams[g]) (void)hipStreamDestroy(e->streams[g]);
-  }
-  delete e;
-  return PS_OK;
-}
-
-const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "null engine"; }
-
-int ps_set_precision(ps_engine *e, int precision) {
-  if (!e) return PS_ERR_ARG;
-  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16X3) {
-    e->err = "unknown precision";
-    return PS_ERR_ARG;
-  }
-  e->precision = precision;
-  return PS_OK;
-}
-
-int ps_set_slots(ps_engine *e, int max_slots) {
-  if (!e || max_slots < 1) return PS_ERR_ARG;
-  e->max_slots = max_slots;
-  return PS_OK;
-}
-
-int ps_set_problem(ps_engine *e, const ps_problem *p) {
-  if (!e || !p) return PS_ERR_ARG;
-  HIP_TRY(hipSetDevice(e->device));
-  HIP_TRY(hipStreamSynchronize(e->stream));
-  free_problem(e);
+// Validates one problem and uploads its tables; D receives the device view.
+// nb_tab: column blocks of 16 for the transposed Dsph tables (the largest
+// problem of a batch, so one kernel instance serves every problem).
+int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D, float *feat) {
+  if (!p) return PS_ERR_ARG;
   const int N = p->n_points, F = p->n_fac;
   if (N < 1 || p->n_lc < 1 || F < 1 || F > kMaxFacets || p->lmax < 0 || p->mmax < 0 || p->mmax > p->lmax ||
       p->n_poles < 1 || !p->lc_points || !p->lc_inrel || !p->tim || !p->brightness || !p->sig || !p->weight ||
@@ -2017,17 +1998,7 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
     return PS_ERR_LIMIT;
   }
   const int ncp = (ncoef + 7) / 8 * 8;
-  const int nb = (ncoef + 15) / 16;
-  const bool mixed = e->precision != PS_PREC_FP64;
-  const int mode = e->precision == PS_PREC_BF16 ? 1 : (e->precision == PS_PREC_BF16X3 ? 2 : 0);
-  const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs)
-  e->use_mfma = mixed || !(kv && !strcmp(kv, "valu"));
-  ModelFn mk = !mixed ? (e->use_mfma ? pick_mfma(ncoef) : pick_model(ncp))
-                      : (mode == 1 ? pick_mixed<1>(ncoef) : (mode == 2 ? pick_mixed<2>(ncoef) : pick_mixed<0>(ncoef)));
-  if (!mk) {
-    e->err = "no model kernel for this coefficient count";
-    return PS_ERR_LIMIT;
-  }
+  const int nb = nb_tab;
   // host-side SoA staging
   std::vector<double> ex(N), ey(N), ez(N), sx(N), sy(N), sz(N);
   for (int i = 0; i < N; i++) {
@@ -2061,8 +2032,8 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
     sa += acos(c);
     sb += fabs(asin(ee[2] > 1 ? 1 : (ee[2] < -1 ? -1 : ee[2])));
   }
-  e->feat_alpha = (float)(sa / N);
-  e->feat_aspect = (float)(sb / N);
+  feat[0] = (float)(sa / N);
+  feat[1] = (float)(sb / N);
   std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
   std::vector<double4> nor4(F);
   std::vector<float4> nor4f(F);
@@ -2104,7 +2075,6 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
       cfs[(size_t)m * F + f] = p->fs[(size_t)f * M1 + m];
     }
   }
-  DevProblem &D = e->P;
   D = DevProblem{};
   int rc = PS_OK;
 #define UP(src, n, dst) \
@@ -2174,20 +2144,181 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
   D.npad = (N + 3 + 63) / 64 * 64;
   D.ntiles = (N + kThreads - 1) / kThreads;
   D.ntiles64 = (N + kTilePts - 1) / kTilePts;
-  e->n_poles = p->n_poles;
+  return PS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps_create(int device, ps_engine **out) {
+  if (!out) return PS_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PS_ERR_HIP;
+  auto *e = new ps_engine();
+  e->device = device;
+  bool ok = hipSetDevice(device) == hipSuccess;
+  for (int g = 0; g < kMaxGroups; g++)
+    ok = ok && hipStreamCreateWithFlags(&e->streams[g], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&e->evg[g], hipEventDisableTiming) == hipSuccess;
+  e->stream = e->streams[0];
+  if (!ok || hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&e->evf, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    delete e;
+    return PS_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete e;
+    return PS_ERR_HIP;
+  }
+  e->cu_count = prop.multiProcessorCount;
+  if (const char *gr = getenv("PSAMD_GRAPH")) e->use_graph = atoi(gr) != 0;
+  if (const char *g = getenv("PSAMD_GROUPS")) {
+    const int v = atoi(g);
+    e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);
+  }
+  *out = e;
+  return PS_OK;
+}
+
+int ps_destroy(ps_engine *e) {
+  if (!e) return PS_ERR_ARG;
+  (void)hipSetDevice(e->device);
+  for (int g = 0; g < kMaxGroups; g++)
+    if (e->streams[g]) (void)hipStreamSynchronize(e->streams[g]);
+  free_problem(e);
+  void *bufs[] = {e->mem, e->st, e->counters, e->d_freq, e->d_pole, e->d_out, e->d_rows, e->d_feat, e->d_cgo,
+                  e->d_iprob, e->d_conw,
+                  e->d_cg, e->d_ea, e->d_eb, e->d_ec, e->d_ek};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (e->host_done) (void)hipHostFree(e->host_done);
+  for (auto ev : e->pev) (void)hipEventDestroy(ev);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->evf) (void)hipEventDestroy(e->evf);
+  for (int g = 0; g < kMaxGroups; g++) {
+    if (e->evg[g]) (void)hipEventDestroy(e->evg[g]);
+    if (e->streams[g]) (void)hipStreamDestroy(e->streams[g]);
+  }
+  delete e;
+  return PS_OK;
+}
+
+const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int ps_set_precision(ps_engine *e, int precision) {
+  if (!e) return PS_ERR_ARG;
+  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16X3) {
+    e->err = "unknown precision";
+    return PS_ERR_ARG;
+  }
+  e->precision = precision;
+  return PS_OK;
+}
+
+int ps_set_slots(ps_engine *e, int max_slots) {
+  if (!e || max_slots < 1) return PS_ERR_ARG;
+  e->max_slots = max_slots;
+  return PS_OK;
+}
+
+int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
+  if (!e || !p || n < 1) return PS_ERR_ARG;
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  free_problem(e);
+  // the largest coefficient count decides the kernel instances and tables
+  int nc_max = 0;
+  for (int i = 0; i < n; i++) {
+    if (p[i].lmax < 0 || p[i].mmax < 0 || p[i].mmax > p[i].lmax) {
+      e->err = "invalid problem";
+      return PS_ERR_ARG;
+    }
+    int nc = 0;
+    for (int m = 0; m <= p[i].mmax; m++) nc += (p[i].lmax - m + 1) * (m ? 2 : 1);
+    nc_max = nc > nc_max ? nc : nc_max;
+  }
+  const int nb = (nc_max + 15) / 16;
+  std::vector<DevProblem> hp(n);
+  DevProblem G{};  // maxima over the batch (grids, slot size, LDS)
+  int rc;
+  for (int i = 0; i < n; i++) {
+    float feat[2];
+    if ((rc = upload_problem(e, p + i, nb, hp[i], feat))) {
+      free_problem(e);
+      return rc;
+    }
+    if (i == 0) {
+      e->feat_alpha = feat[0];
+      e->feat_aspect = feat[1];
+      G = hp[0];
+    }
+    const DevProblem &D = hp[i];
+    G.nfac = std::max(G.nfac, D.nfac);
+    G.ncoef = std::max(G.ncoef, D.ncoef);
+    G.ncp = std::max(G.ncp, D.ncp);
+    G.ma = std::max(G.ma, D.ma);
+    G.mfit = std::max(G.mfit, D.mfit);
+    G.npad = std::max(G.npad, D.npad);
+    G.n_lc = std::max(G.n_lc, D.n_lc);
+    G.ntiles = std::max(G.ntiles, D.ntiles);
+    G.ntiles64 = std::max(G.ntiles64, D.ntiles64);
+    G.n_iter_max = std::max(G.n_iter_max, D.n_iter_max);
+    G.n_iter_min = std::max(G.n_iter_min, D.n_iter_min);
+  }
+  const int F = G.nfac, ncoef = G.ncoef, mfit = G.mfit;
+  const bool mixed = e->precision != PS_PREC_FP64;
+  const int mode = e->precision == PS_PREC_BF16 ? 1 : (e->precision == PS_PREC_BF16X3 ? 2 : 0);
+  const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs, one problem)
+  e->use_mfma = mixed || n > 1 || !(kv && !strcmp(kv, "valu"));
+  const bool multi = n > 1;
+  ModelFn mk;
+  if (!mixed)
+    mk = !e->use_mfma ? pick_model<false>(G.ncp) : (multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef));
+  else if (multi)
+    mk = mode == 1 ? pick_mixed<1, true>(ncoef) : (mode == 2 ? pick_mixed<2, true>(ncoef) : pick_mixed<0, true>(ncoef));
+  else
+    mk = mode == 1 ? pick_mixed<1, false>(ncoef) : (mode == 2 ? pick_mixed<2, false>(ncoef) : pick_mixed<0, false>(ncoef));
+  if (!mk) {
+    free_problem(e);
+    e->err = "no model kernel for this coefficient count";
+    return PS_ERR_LIMIT;
+  }
+  const DevProblem *dp = nullptr;
+  if ((rc = upload(e, hp.data(), (size_t)n, &dp)) != PS_OK) {
+    free_problem(e);
+    return rc;
+  }
+  e->d_probs = dp;
+  e->P = hp[0];
+  e->G = G;
+  e->n_problems = n;
+  e->n_poles = p[0].n_poles;
+  e->prob_npoles.resize(n);
+  for (int i = 0; i < n; i++) e->prob_npoles[i] = p[i].n_poles;
   e->model = mk;
   e->lds_step = step_lds(mfit, F);
   e->lds_model = mixed ? mixed_layout(F, nb, mode).total : (e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
   e->lds_neq = neq_lds(mfit, mixed);
   const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
   if (e->lds_step > lds_max || e->lds_neq > lds_max || e->lds_model > lds_max) {
+    free_problem(e);
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;
   }
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(step_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_step));
-  e->neq = mixed ? pick_neq<true>(mfit) : pick_neq<false>(mfit);
+  e->step = multi ? step_kernel<true> : step_kernel<false>;
+  e->curv = multi ? curv_kernel<true> : curv_kernel<false>;
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(e->step), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)e->lds_step));
+  e->neq = mixed ? (multi ? pick_neq<true, true>(mfit) : pick_neq<true, false>(mfit))
+                 : (multi ? pick_neq<false, true>(mfit) : pick_neq<false, false>(mfit));
   if (!e->neq) {
+    free_problem(e);
     e->err = "too many free parameters";
     return PS_ERR_LIMIT;
   }
@@ -2197,6 +2328,11 @@ int ps_set_problem(ps_engine *e, const ps_problem *p) {
                               (int)e->lds_model));
   e->have_problem = true;
   return PS_OK;
+}
+
+int ps_set_problem(ps_engine *e, const ps_problem *p) {
+  if (!e || !p) return PS_ERR_ARG;
+  return ps_set_problems(e, p, 1);
 }
 
 int ps_dims(const ps_engine *e, int *ncoef, int *ma, int *mfit) {
@@ -2311,6 +2447,50 @@ int ps_precalc(ps_engine *e, double freq_start, double freq_end, double freq_ste
   *new_conw = 0;
   if (ave_dark < 1.0) *new_conw = 1;
   if (ave_dark >= 1.0) *conw_r = *conw_r * 2;
+  return PS_OK;
+}
+
+int ps_fit_batch_multi(ps_engine *e, const int *prob, const double *freq, const int *pole, int n,
+                       const double *conw_r, ps_fit_out *out) {
+  int rc = check_ready(e, false);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!prob || !freq || !pole || !conw_r || !out))) return PS_ERR_ARG;
+  if (n == 0) return PS_OK;
+  for (int i = 0; i < n; i++)
+    if (prob[i] < 0 || prob[i] >= e->n_problems || pole[i] < 0 || pole[i] >= e->prob_npoles[prob[i]]) {
+      e->err = "problem or pole index out of range";
+      return PS_ERR_ARG;
+    }
+  if ((rc = ensure_items(e, n))) return rc;
+  if ((size_t)n > e->iprob_cap) {
+    if (e->d_iprob) (void)hipFree(e->d_iprob);
+    e->d_iprob = nullptr;
+    e->iprob_cap = 0;
+    HIP_TRY(hipMalloc(&e->d_iprob, (size_t)n * sizeof(int)));
+    e->iprob_cap = n;
+  }
+  if ((size_t)e->n_problems > e->conw_cap) {
+    if (e->d_conw) (void)hipFree(e->d_conw);
+    e->d_conw = nullptr;
+    e->conw_cap = 0;
+    HIP_TRY(hipMalloc(&e->d_conw, (size_t)e->n_problems * sizeof(double)));
+    e->conw_cap = e->n_problems;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_freq, freq, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_pole, pole, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_iprob, prob, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_conw, conw_r, e->n_problems * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  DevWork W{};
+  W.n_items = n;
+  W.mode = 0;
+  W.freq = e->d_freq;
+  W.pole = e->d_pole;
+  W.item_prob = e->d_iprob;
+  W.conw_rs = e->d_conw;
+  W.out = e->d_out;
+  if ((rc = run_batch(e, W))) return rc;
+  HIP_TRY(hipMemcpyAsync(out, e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
   return PS_OK;
 }
 
