@@ -179,6 +179,10 @@ int ps_dims(const ps_engine *e, int *ncoef, int *ma, int *mfit);
 /* Time of the last fit-kernel launch [ms] measured with HIP events on the
  * engine's stream, and the number of fits it ran. */
 int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits);
+/* The last batch of fits: GPU time (ms), fits, and LM iterations run over the
+ * slots (every resident fit advances one iteration per LM iteration of the
+ * batch).  No counterpart in the reference. */
+int ps_last_batch(const ps_engine *e, double *ms, long long *items, long long *iterations);
 
 /* Per-launch timing of the model kernel (the dominant kernel) with HIP
  * events on the engine's stream, and a count of model evaluations.

@@ -90,6 +90,7 @@ def _load() -> C.CDLL:
         "ps_eval_mrqcof": (C.c_int, [vp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
         "ps_dims": (C.c_int, [vp, _ip, _ip, _ip]),
         "ps_last_kernel_ms": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
+        "ps_last_batch": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ps_set_profiling": (C.c_int, [vp, C.c_int]),
         "ps_kernel_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ps_input_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
@@ -122,7 +123,7 @@ lib = _load()
 
 # Symbols declared in include/*.h (checked by the CPU test suite).
 EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_set_slots", "ps_set_problem",
-            "ps_set_problems", "ps_fit_batch_multi",
+            "ps_set_problems", "ps_fit_batch_multi", "ps_last_batch",
             "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
             "ps_input_scalars", "ps_format_row", "ps_shape_basis",
@@ -318,6 +319,14 @@ class Engine:
         ev = C.c_longlong()
         lib.ps_kernel_stats(self._h, C.byref(ms), C.byref(n), C.byref(ev))
         return dict(model_ms=ms.value, model_launches=n.value, evals=ev.value)
+
+    def last_batch(self):
+        """(GPU ms, fits, LM iterations) of the last batch of fits."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        it = C.c_longlong()
+        lib.ps_last_batch(self._h, C.byref(ms), C.byref(n), C.byref(it))
+        return ms.value, n.value, it.value
 
     def last_kernel_ms(self):
         ms = C.c_double()

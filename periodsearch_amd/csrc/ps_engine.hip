@@ -2350,6 +2350,14 @@ int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits) {
   return PS_OK;
 }
 
+int ps_last_batch(const ps_engine *e, double *ms, long long *items, long long *iterations) {
+  if (!e) return PS_ERR_ARG;
+  if (ms) *ms = e->last_ms;
+  if (items) *items = e->last_items;
+  if (iterations) *iterations = e->last_iters;
+  return PS_OK;
+}
+
 int ps_set_profiling(ps_engine *e, int on) {
   if (!e) return PS_ERR_ARG;
   e->profile = on != 0;

@@ -186,3 +186,22 @@ def write_workload(name: str, path: str, **overrides) -> dict:
     ast = make_asteroid(cfg.pop("seed"), **cfg)
     write_input(path, ast, per_start, per_end, stop=stop)
     return ast
+
+
+def uneven_asteroid(seed: int) -> dict:
+    """Parameters of one asteroid of the uneven multi-asteroid batch: point
+    count log-uniform in [200, 8000] (about 30% in dense light curves of
+    20-150 points, the rest sparse), lmax = mmax uniform in {4, 6, 8} with
+    nrows 6 / 8 / 10, period log-uniform in [2, 50] h, a random pole, 2-5%
+    noise, 4-15 years, orbit inclination 0-15 deg."""
+    rng = np.random.default_rng(seed)
+    n = int(round(np.exp(rng.uniform(np.log(200), np.log(8000)))))
+    lmax = int(rng.choice([4, 6, 8]))
+    n_dense = int(min(20, max(1, n // 400)))
+    dense_pts = int(min(150, max(20, 0.3 * n / n_dense)))
+    return dict(seed=seed, lmax=lmax, mmax=lmax, nrows={4: 6, 6: 8, 8: 10}[lmax], n_dense=n_dense,
+                dense_pts=dense_pts, n_sparse=max(0, n - n_dense * dense_pts),
+                noise=float(rng.uniform(0.02, 0.05)), span_years=float(rng.uniform(4, 15)),
+                period_h=float(np.exp(rng.uniform(np.log(2.0), np.log(50.0)))),
+                pole=(float(rng.uniform(0, 360)), float(np.degrees(np.arcsin(rng.uniform(-1, 1))))),
+                incl_deg=float(rng.uniform(0, 15)))
