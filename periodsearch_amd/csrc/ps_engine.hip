@@ -377,13 +377,20 @@ __device__ void model_tile(const DevProblem &P, int p0, const double *a, const d
 }
 
 // The three convexity rows (period_search/conv.c:12-29) at points
-// n_points..n_points+2, and chck = sum Area*Nor.  Serial over facets in the
-// reference order.  MIX: derivative rows stored as FP32 (the model value row
-// ma stays FP64).
+// n_points..n_points+2, and chck = sum Area*Nor.  Each sum over facets is
+// split into kConvParts interleaved parts (facets f = part + kConvParts*i,
+// in order), summed by separate threads with their loads issued eight
+// facets at a time, and the parts are added in order; consecutive threads
+// take consecutive columns, so the Dsph rows are read coalesced.  red:
+// LDS scratch of kConvParts * (3 * ncoef + 3) doubles.  MIX: derivative rows
+// stored as FP32 (the model value row ma stays FP64).
+constexpr int kConvParts = 4;
+
 template <bool MIX>
-__device__ void conv_rows(const DevProblem &P, const double *area, const double *g, double *J, double *chck) {
+__device__ void conv_rows(const DevProblem &P, const double *area, const double *g, double *J, double *chck,
+                          double *red) {
 #pragma clang fp contract(off)
-  const int tid = threadIdx.x, ld = P.npad, n = P.n_points;
+  const int tid = threadIdx.x, ld = P.npad, n = P.n_points, F = P.nfac, nc = P.ncoef;
   float *Jf = reinterpret_cast<float *>(J);
   auto put = [&](int c, int q, double v) {
     if (MIX)
@@ -391,55 +398,73 @@ __device__ void conv_rows(const DevProblem &P, const double *area, const double 
     else
       J[(size_t)c * ld + q] = v;
   };
-  // facets in the reference order; loads issued eight facets at a time
-  const int F = P.nfac;
-  if (tid < P.ncoef) {
+  // tasks: (part, column) for the shape columns, then (part, k) for chck
+  const int ntask = kConvParts * (nc + 3);
+  for (int t = tid; t < ntask; t += blockDim.x) {
+    const int part = t / (nc + 3), c = t - part * (nc + 3);
     double r0 = 0, r1 = 0, r2 = 0;
-    for (int f0 = 0; f0 < F; f0 += 8) {
-      double gv[8], dv[8];
+    if (c < nc) {
+      for (int f0 = part; f0 < F; f0 += 8 * kConvParts) {
+        double gv[8], dv[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int f = min(f0 + u, F - 1);
-        gv[u] = g[f];
-        dv[u] = P.dsph[(size_t)f * P.ncp + tid];
-      }
+        for (int u = 0; u < 8; u++) {
+          const int f = min(f0 + kConvParts * u, F - 1);
+          gv[u] = g[f];
+          dv[u] = P.dsph[(size_t)f * P.ncp + c];
+        }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int f = f0 + u;
-        if (f < F) {
-          double dg = gv[u] * dv[u];
-          double t = P.darea[f] * dg;
-          r0 += t * P.nx[f];
-          r1 += t * P.ny[f];
-          r2 += t * P.nz[f];
+        for (int u = 0; u < 8; u++) {
+          const int f = f0 + kConvParts * u;
+          if (f < F) {
+            const double dg = gv[u] * dv[u];
+            const double tt = P.darea[f] * dg;
+            r0 += tt * P.nx[f];
+            r1 += tt * P.ny[f];
+            r2 += tt * P.nz[f];
+          }
         }
       }
-    }
-    put(tid, n, r0);
-    put(tid, n + 1, r1);
-    put(tid, n + 2, r2);
-  } else if (tid < P.ma) {
-    put(tid, n, 0);
-    put(tid, n + 1, 0);
-    put(tid, n + 2, 0);
-  } else if (tid >= kThreads - 3 && tid < kThreads) {
-    const int nc = tid - (kThreads - 3);
-    const double *nv = nc == 0 ? P.nx : (nc == 1 ? P.ny : P.nz);
-    double r = 0;
-    for (int f0 = 0; f0 < F; f0 += 8) {
-      double av[8], nn[8];
+      red[(size_t)part * (3 * nc + 3) + 3 * c] = r0;
+      red[(size_t)part * (3 * nc + 3) + 3 * c + 1] = r1;
+      red[(size_t)part * (3 * nc + 3) + 3 * c + 2] = r2;
+    } else {
+      const int k = c - nc;
+      const double *nv = k == 0 ? P.nx : (k == 1 ? P.ny : P.nz);
+      double r = 0;
+      for (int f0 = part; f0 < F; f0 += 8 * kConvParts) {
+        double av[8], nn[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int f = min(f0 + u, F - 1);
-        av[u] = area[f];
-        nn[u] = nv[f];
+        for (int u = 0; u < 8; u++) {
+          const int f = min(f0 + kConvParts * u, F - 1);
+          av[u] = area[f];
+          nn[u] = nv[f];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (f0 + kConvParts * u < F) r += av[u] * nn[u];
       }
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (f0 + u < F) r += av[u] * nn[u];
+      red[(size_t)part * (3 * nc + 3) + 3 * nc + k] = r;
     }
-    J[(size_t)P.ma * ld + n + nc] = r;
-    chck[nc] = r;
+  }
+  __syncthreads();
+  for (int t = tid; t < P.ma + 3; t += blockDim.x) {
+    if (t < nc) {
+      for (int k = 0; k < 3; k++) {
+        double v = red[3 * t + k];
+        for (int part = 1; part < kConvParts; part++) v = v + red[(size_t)part * (3 * nc + 3) + 3 * t + k];
+        put(t, n + k, v);
+      }
+    } else if (t < P.ma) {
+      put(t, n, 0);
+      put(t, n + 1, 0);
+      put(t, n + 2, 0);
+    } else {
+      const int k = t - P.ma;
+      double v = red[3 * nc + k];
+      for (int part = 1; part < kConvParts; part++) v = v + red[(size_t)part * (3 * nc + 3) + 3 * nc + k];
+      J[(size_t)P.ma * ld + n + k] = v;
+      chck[k] = v;
+    }
   }
 }
 
@@ -458,11 +483,14 @@ __device__ void conv_rows(const DevProblem &P, const double *area, const double 
 // light-curve sums, beta, chi2 and the residuals stay FP64.
 constexpr int kNeqChunk = 64;
 
-// Doubles taken by the staged rows Jn (also the light-curve-sum scratch).
+// Doubles taken by the staged rows Jn (also the light-curve-sum and the
+// convexity-row scratch; ncoef <= mfit + 1).
 __host__ __device__ inline int neq_jn_doubles(int mfit, bool mix) {
   const int MS = (mfit + 15) / 16 * 16 + 1;
   int d = mix ? (kNeqChunk * MS + 1) / 2 : kNeqChunk * MS;
-  return d < 4 * (kMaxMa + 1) ? 4 * (kMaxMa + 1) : d;
+  if (d < 4 * (kMaxMa + 1)) d = 4 * (kMaxMa + 1);
+  if (d < kConvParts * (3 * (mfit + 1) + 3)) d = kConvParts * (3 * (mfit + 1) + 3);
+  return d;
 }
 
 template <int NT, bool MIX>
@@ -1407,7 +1435,7 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevP
   if (eval) {
     if (tid == 0) atomicAdd(W.done + 1, 1);
     for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
-    conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1);
+    conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1, wst);
     __threadfence_block();
     __syncthreads();
     normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, btr,
