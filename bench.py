@@ -157,6 +157,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=("wu", "batch64"), default="wu",
                     help="batch64: 64 uneven asteroids in one batch of fits (plain start)")
     ap.add_argument("--window", type=int, default=16, help="batch64: trial periods per asteroid per step")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time slot counts and slot groups on this workload first and keep the fastest")
     ap.add_argument("--candidates", type=int, default=4, help="learned form: periods verified per block")
     ap.add_argument("--block", type=int, default=1024, help="learned form: trial periods per block")
     args = ap.parse_args()
@@ -191,6 +193,12 @@ def main() -> int:
     if args.slots:
         eng.set_slots(args.slots)
     eng.set_problem(inp.problem, keep=inp)
+    tuned = None
+    if args.autotune:
+        from periodsearch_amd.autotune import autotune
+
+        # periods far from the timed ones (the same per-fit cost under stop 50)
+        tuned = autotune(eng, inp, inp.conw_r0, n_from=inp.n_periods // 2)
     nc, ma, mfit = eng.dims()
     N, F = inp.ndata, inp.problem.n_fac
     conw_r = inp.conw_r0
@@ -265,6 +273,8 @@ def main() -> int:
         "job_tflops": round(lm_fits * flop_fit / wall / 1e12, 3),
         "lm_fits_per_s": round(lm_fits / wall, 2),
     }
+    if tuned:
+        out["config"]["autotuned"] = {"slots": tuned["best"]["slots"], "groups": tuned["best"]["groups"]}
     if args.form == "learned":
         out["config"]["learned"] = (f"1 proposed start per trial period; in each {args.block}-period block the "
                                     f"{args.candidates} best periods and 2 neighbours each refitted from the 10 "

@@ -151,6 +151,11 @@ int ps_fit_batch_multi(ps_engine *e, const int *prob, const double *freq, const 
  * slot holds one fit's LM state and Jacobian in HBM. */
 int ps_set_slots(ps_engine *e, int max_slots);
 
+/* Slot groups, 1..4 (default 1): the slots are split into groups, each
+ * driven on its own HIP stream, so the per-slot kernels of one group overlap
+ * the model kernels of the others.  Results do not depend on it. */
+int ps_set_groups(ps_engine *e, int groups);
+
 /* One convexity-weight round (CUDAPrecalc): fits min(10, nPeriods) periods
  * x n_poles at conw_r, averages the best dark-facet %, sets *new_conw = 1 if
  * the average is < 1, else doubles *conw_r.  Frequencies are 1/P with P

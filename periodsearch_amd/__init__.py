@@ -81,6 +81,7 @@ def _load() -> C.CDLL:
         "ps_last_error": (C.c_char_p, [vp]),
         "ps_set_precision": (C.c_int, [vp, C.c_int]),
         "ps_set_slots": (C.c_int, [vp, C.c_int]),
+        "ps_set_groups": (C.c_int, [vp, C.c_int]),
         "ps_set_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
         "ps_set_problems": (C.c_int, [vp, C.POINTER(PsProblem), C.c_int]),
         "ps_fit_batch_multi": (C.c_int, [vp, _ip, _dp, _ip, C.c_int, _dp, vp]),
@@ -123,7 +124,7 @@ lib = _load()
 
 # Symbols declared in include/*.h (checked by the CPU test suite).
 EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_set_slots", "ps_set_problem",
-            "ps_set_problems", "ps_fit_batch_multi", "ps_last_batch",
+            "ps_set_problems", "ps_fit_batch_multi", "ps_last_batch", "ps_set_groups",
             "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
             "ps_input_scalars", "ps_format_row", "ps_shape_basis",
@@ -229,6 +230,9 @@ class Engine:
 
     def set_slots(self, n: int):
         self._check(lib.ps_set_slots(self._h, n), "ps_set_slots")
+
+    def set_groups(self, n: int):
+        self._check(lib.ps_set_groups(self._h, n), "ps_set_groups")
 
     def set_problem(self, problem: PsProblem, keep=None):
         self._keep = keep  # keep the arrays behind the pointers alive

@@ -2249,6 +2249,12 @@ int ps_set_precision(ps_engine *e, int precision) {
   return PS_OK;
 }
 
+int ps_set_groups(ps_engine *e, int groups) {
+  if (!e || groups < 1 || groups > kMaxGroups) return PS_ERR_ARG;
+  e->groups = groups;
+  return PS_OK;
+}
+
 int ps_set_slots(ps_engine *e, int max_slots) {
   if (!e || max_slots < 1) return PS_ERR_ARG;
   e->max_slots = max_slots;
