@@ -83,17 +83,31 @@ def bench_batch(args, prec) -> int:
     from periodsearch_amd import Engine, Input, synth
 
     tmp = tempfile.mkdtemp(prefix="psbatch_")
-    inps, ns = [], []
+    paths, periods = [], []
     for i in range(64):
         cfg = synth.uneven_asteroid(1000 + i)
         ast = synth.make_asteroid(**cfg)
-        P = cfg["period_h"]
-        path = os.path.join(tmp, f"in{i}")
-        synth.write_input(path, ast, P * 0.98, P * 1.02)
-        inp = Input(path)
-        inps.append(inp)
-        ns.append(int(round((inp.freq_start - 1.0 / P) / inp.freq_step)) + 1)
+        paths.append(os.path.join(tmp, f"in{i}"))
+        periods.append(cfg["period_h"])
+        synth.write_input(paths[-1], ast, periods[-1] * 0.98, periods[-1] * 1.02)
     eng = Engine(0)
+    # shape setup of the 64 asteroids: host (ps_input_read builds each basis)
+    # against the GPU (deferred reads, then one ps_shape_basis_many call)
+    t0 = time.perf_counter()
+    host_inps = [Input(p) for p in paths]
+    t_host = time.perf_counter() - t0
+    eng.shape_basis_inputs([Input(paths[0], defer_shape=True)])  # module load, first allocation
+    t0 = time.perf_counter()
+    inps = [Input(p, defer_shape=True) for p in paths]
+    t_read = time.perf_counter() - t0
+    eng.shape_basis_inputs(inps)
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(3):
+        [Input(p, defer_shape=True) for p in paths]
+    t_parse = (time.perf_counter() - t0) / 3
+    del host_inps
+    ns = [int(round((inp.freq_start - 1.0 / P) / inp.freq_step)) + 1 for inp, P in zip(inps, periods)]
     eng.set_precision(prec)
     eng.set_problems([inp.problem for inp in inps], keep=inps)
     conw = [inp.conw_r0 for inp in inps]
@@ -136,6 +150,10 @@ def bench_batch(args, prec) -> int:
                            "p99": round(float(np.percentile(lat, 99)), 2),
                            "note": "(LM iterations + 2) x GPU time per LM iteration of the batch"},
         "model_evals": st["evals"],
+        "shape_setup_ms": {"host": round((t_host - t_parse) * 1e3, 2), "gpu": round((t_gpu - t_read) * 1e3, 2),
+                           "asteroids": 64, "shapes": len({(i.problem.n_fac, i.problem.lmax) for i in inps}),
+                           "note": "shape basis of all 64 asteroids beyond parsing their inputs: host ps_shape_basis"
+                                   " per asteroid vs one ps_shape_basis_many call"},
     }
     print(json.dumps(res), flush=True)
     eng.close()

@@ -216,6 +216,32 @@ int ps_format_row(const ps_row *r, int round_pole, char *buf, int buflen);
 int ps_shape_basis(int nrows, int lmax, int mmax, double a, double b, double c, double *nor, double *darea,
                    double *pleg, double *fc, double *fs, double *dsph, double *cg_first);
 
+/* One asteroid's request for ps_shape_basis_many: triangulation rows,
+ * Laplace-series degree and order, ellipsoid axes, and host buffers for the
+ * tables, sized as for ps_shape_basis. */
+typedef struct ps_shape_job {
+  int nrows, lmax, mmax;
+  double a, b, c;
+  double *nor, *darea, *pleg, *fc, *fs, *dsph, *cg_first;
+} ps_shape_job;
+
+/* Shape setup of n asteroids on the engine's GPU: the work of n
+ * ps_shape_basis calls (trifac, areanorm, sphfunc, ellfit as called at
+ * period_search_BOINC.cpp:507-552) in six launches.  The facet tables and
+ * the ellipsoid normal matrix and its LU are built once per distinct
+ * (nrows, lmax, mmax); per asteroid remain the ellipsoid's log-curvature, one
+ * right-hand side and its substitution.  The arithmetic follows
+ * ps_shape_basis's order; results differ only where the device's
+ * sin/cos/acos/atan2/log round differently from the host's (<= 1 ulp).  Does
+ * not touch the engine's problem.  Returns PS_OK or <0. */
+int ps_shape_basis_many(ps_engine *e, int n, const ps_shape_job *jobs);
+
+/* ps_input_read without the shape basis: the tables are sized and zero until
+ * ps_shape_basis_many fills them through ps_input_shape_job's pointers. */
+int ps_input_read_deferred(const char *path, ps_input **out);
+/* The input's shape request, pointing at its own table storage. */
+int ps_input_shape_job(ps_input *in, ps_shape_job *job);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,13 @@ class PsProblem(C.Structure):
     ]
 
 
+class PsShapeJob(C.Structure):
+    _fields_ = [("nrows", C.c_int), ("lmax", C.c_int), ("mmax", C.c_int),
+                ("a", C.c_double), ("b", C.c_double), ("c", C.c_double),
+                ("nor", _dp), ("darea", _dp), ("pleg", _dp), ("fc", _dp), ("fs", _dp), ("dsph", _dp),
+                ("cg_first", _dp)]
+
+
 class PsFitOut(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dev", "chisq", "rchisq", "prd", "la", "be", "dark", "totarea", "alamda")] + [
         ("niter", C.c_int), ("err", C.c_int)]
@@ -95,6 +102,9 @@ def _load() -> C.CDLL:
         "ps_set_profiling": (C.c_int, [vp, C.c_int]),
         "ps_kernel_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ps_input_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "ps_input_read_deferred": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "ps_input_shape_job": (C.c_int, [vp, C.POINTER(PsShapeJob)]),
+        "ps_shape_basis_many": (C.c_int, [vp, C.c_int, C.POINTER(PsShapeJob)]),
         "ps_input_free": (None, [vp]),
         "ps_input_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
         "ps_input_scalars": (C.c_int, [vp, _dp, _ip]),
@@ -127,7 +137,8 @@ EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_
             "ps_set_problems", "ps_fit_batch_multi", "ps_last_batch", "ps_set_groups",
             "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
-            "ps_input_scalars", "ps_format_row", "ps_shape_basis",
+            "ps_input_scalars", "ps_format_row", "ps_shape_basis", "ps_shape_basis_many", "ps_input_read_deferred",
+            "ps_input_shape_job",
             "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
             "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
             "ps_mlp_train_step")
@@ -172,12 +183,23 @@ def shape_basis(nrows: int, lmax: int, mmax: int, a: float, b: float, c: float) 
     return out
 
 
-class Input:
-    """A parsed period_search_in (period_search/period_search_BOINC.cpp:277-485)."""
+def _shape_arrays(nrows: int, lmax: int, mmax: int) -> dict:
+    F = 8 * nrows * nrows
+    return dict(nor=np.zeros(F * 3), darea=np.zeros(F), pleg=np.zeros(F * (lmax + 1) ** 2),
+                fc=np.zeros(F * (mmax + 1)), fs=np.zeros(F * (mmax + 1)), dsph=np.zeros(F * n_coef(lmax, mmax)),
+                cg_first=np.zeros(n_coef(lmax, mmax)))
 
-    def __init__(self, path: str):
+
+class Input:
+    """A parsed period_search_in (period_search/period_search_BOINC.cpp:277-485).
+
+    With ``defer_shape`` the shape basis is left for ``Engine.shape_basis_inputs``
+    (the GPU shape setup of many inputs at once)."""
+
+    def __init__(self, path: str, defer_shape: bool = False):
         h = C.c_void_p()
-        rc = lib.ps_input_read(os.fsencode(path), C.byref(h))
+        reader = lib.ps_input_read_deferred if defer_shape else lib.ps_input_read
+        rc = reader(os.fsencode(path), C.byref(h))
         if rc != 0:
             raise PsError(f"cannot read {path}: {rc}")
         self._h = h
@@ -337,6 +359,34 @@ class Engine:
         n = C.c_longlong()
         lib.ps_last_kernel_ms(self._h, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+    def shape_basis_many(self, specs) -> list:
+        """GPU shape setup (ps_shape_basis_many): specs is a list of
+        (nrows, lmax, mmax, a, b, c); returns one dict per spec shaped as
+        shape_basis's."""
+        outs, jobs = [], (PsShapeJob * max(len(specs), 1))()
+        for j, (nrows, lmax, mmax, a, b, c) in enumerate(specs):
+            o = _shape_arrays(nrows, lmax, mmax)
+            outs.append(o)
+            jobs[j] = PsShapeJob(nrows, lmax, mmax, a, b, c, *(_d(o[k]) for k in
+                                                             ("nor", "darea", "pleg", "fc", "fs", "dsph", "cg_first")))
+        self._check(lib.ps_shape_basis_many(self._h, len(specs), jobs), "ps_shape_basis_many")
+        for (nrows, lmax, mmax, *_), o in zip(specs, outs):
+            F = 8 * nrows * nrows
+            o["nor"] = o["nor"].reshape(F, 3)
+            o["dsph"] = o["dsph"].reshape(F, n_coef(lmax, mmax))
+            o["pleg"] = o["pleg"].reshape(F, lmax + 1, lmax + 1)
+            o["fc"] = o["fc"].reshape(F, mmax + 1)
+            o["fs"] = o["fs"].reshape(F, mmax + 1)
+        return outs
+
+    def shape_basis_inputs(self, inputs) -> None:
+        """Fill the shape tables of inputs read with defer_shape=True, all in
+        one ps_shape_basis_many call."""
+        jobs = (PsShapeJob * max(len(inputs), 1))()
+        for j, inp in enumerate(inputs):
+            self._check(lib.ps_input_shape_job(inp._h, C.byref(jobs[j])), "ps_input_shape_job")
+        self._check(lib.ps_shape_basis_many(self._h, len(inputs), jobs), "ps_shape_basis_many")
 
 
 class Mlp:

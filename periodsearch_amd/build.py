@@ -1,7 +1,8 @@
 """Build recipe for the engine library and the command-line driver.
 
 Compiles, for gfx950 only:
-  periodsearch_amd/libpsamd.so        -- csrc/ps_engine.hip + csrc/ps_mlp.hip + csrc/ps_host.cpp
+  periodsearch_amd/libpsamd.so        -- csrc/ps_engine.hip + csrc/ps_mlp.hip + csrc/ps_shape.hip
+                                         + csrc/ps_host.cpp
                                          (C ABI of include/periodsearch.h and include/ps_proposal.h)
   periodsearch_amd/period_search_amd  -- csrc/ps_main.cpp linked against libpsamd.so
 
@@ -47,15 +48,18 @@ def build(verbose: bool = False, defines=(), lib_out: str | None = None, driver:
     eng_o = os.path.join(HERE, "_obj", f"ps_engine{tag}.o")
     host_o = os.path.join(HERE, "_obj", "ps_host.o")
     mlp_o = os.path.join(HERE, "_obj", "ps_mlp.o")
+    shape_o = os.path.join(HERE, "_obj", "ps_shape.o")
     _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-I", INCLUDE, *DEVICE_FLAGS,
           *[f"-D{d}" for d in defines], "-c", "-o", eng_o, os.path.join(CSRC, "ps_engine.hip")])
     _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-I", INCLUDE, *DEVICE_FLAGS,
           "-c", "-o", mlp_o, os.path.join(CSRC, "ps_mlp.hip")])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-I", INCLUDE, *DEVICE_FLAGS,
+          "-c", "-o", shape_o, os.path.join(CSRC, "ps_shape.hip")])
     # no sin/cos fusion by the compiler (see csrc/ps_host.cpp)
     _run(["g++", "-O3", "-fPIC", "-std=c++17", "-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sincos",
           "-I", INCLUDE, "-c", "-o", host_o,
           os.path.join(CSRC, "ps_host.cpp")])
-    objs = [eng_o, mlp_o, host_o]
+    objs = [eng_o, mlp_o, shape_o, host_o]
     lib = lib_out or LIB
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "periodsearch.h"
+#include "ps_internal.h"
 #include "ps_proposal.h"
 
 namespace {
@@ -2176,6 +2177,10 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
 }
 
 }  // namespace
+
+void *psamd_engine_stream(ps_engine *e) { return e->stream; }
+int psamd_engine_device(const ps_engine *e) { return e->device; }
+void psamd_engine_error(ps_engine *e, const std::string &msg) { e->err = msg; }
 
 extern "C" {
 

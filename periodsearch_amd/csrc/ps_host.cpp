@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "periodsearch.h"
+#include "ps_internal.h"
 
 namespace {
 
@@ -318,6 +319,16 @@ struct ps_input {
   std::vector<double> beta_pole, lambda_pole;
 };
 
+void psamd_triangulate(int nrows, int *ifp) {
+  std::vector<int> t;
+  triangulate(nrows, t);
+  std::memcpy(ifp, t.data(), t.size() * sizeof(int));
+}
+
+namespace {
+int read_input(const char *path, bool shape, ps_input **out);
+}
+
 extern "C" {
 
 int ps_shape_basis(int nrows, int lmax, int mmax, double a, double b, double c, double *nor, double *darea,
@@ -336,9 +347,37 @@ int ps_shape_basis(int nrows, int lmax, int mmax, double a, double b, double c, 
   return nfac;
 }
 
+int ps_input_read(const char *path, ps_input **out) { return read_input(path, true, out); }
+
+int ps_input_read_deferred(const char *path, ps_input **out) { return read_input(path, false, out); }
+
+int ps_input_shape_job(ps_input *in, ps_shape_job *job) {
+  if (!in || !job) return PS_ERR_ARG;
+  job->nrows = in->nrows;
+  job->lmax = in->lmax;
+  job->mmax = in->mmax;
+  job->a = in->a;
+  job->b = in->b;
+  job->c = in->c;
+  job->nor = in->nor.data();
+  job->darea = in->darea.data();
+  job->pleg = in->pleg.data();
+  job->fc = in->fc.data();
+  job->fs = in->fs.data();
+  job->dsph = in->dsph.data();
+  job->cg_first = in->cg_first.data();
+  return PS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
 // period_search_BOINC.cpp:277-485.  Weights use the checked loop of
 // period_search_optimization_cpu/period_search/period_search_BOINC.cpp:440-450.
-int ps_input_read(const char *path, ps_input **out) {
+// With shape == false the shape tables are sized and left zero, for
+// ps_shape_basis_many to fill.
+int read_input(const char *path, bool shape, ps_input **out) {
   if (!path || !out) return PS_ERR_ARG;
   FILE *f = fopen(path, "r");
   if (!f) return PS_ERR_IO;
@@ -466,14 +505,19 @@ int ps_input_read(const char *path, ps_input **out) {
   in->fs.resize(in->nfac * M);
   in->dsph.resize((size_t)in->nfac * in->ncoef);
   in->cg_first.resize(in->ncoef);
-  ps_shape_basis(in->nrows, in->lmax, in->mmax, in->a, in->b, in->c, in->nor.data(), in->darea.data(),
-                 in->pleg.data(), in->fc.data(), in->fs.data(), in->dsph.data(), in->cg_first.data());
+  if (shape)
+    ps_shape_basis(in->nrows, in->lmax, in->mmax, in->a, in->b, in->c, in->nor.data(), in->darea.data(),
+                   in->pleg.data(), in->fc.data(), in->fs.data(), in->dsph.data(), in->cg_first.data());
   // initial poles (period_search_BOINC.cpp:197-206)
   in->beta_pole = {0, 0, 0, 0, 60, 60, 60, -60, -60, -60};
   in->lambda_pole = {0, 90, 180, 270, 60, 180, 300, 60, 180, 300};
   *out = in;
   return PS_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 void ps_input_free(ps_input *in) { delete in; }
 

@@ -59,3 +59,30 @@ def test_format_row_matches_reference_printf():
     row["lambda"] = 2.5
     assert ps.format_row(row).split()[4] == "2"          # printf rounds half to even
     assert ps.format_row(row, round_pole=True).split()[4] == "3"  # round() as the CUDA/OC apps
+
+
+def test_deferred_input_hands_its_tables_to_the_shape_job(small_input):
+    """ps_input_read_deferred leaves the tables zero and ps_input_shape_job
+    points at them; filling them through the job's pointers gives the
+    eagerly read input's tables (here with the host shape basis)."""
+    eager = ps.Input(small_input)
+    lazy = ps.Input(small_input, defer_shape=True)
+    job = ps.PsShapeJob()
+    assert ps.lib.ps_input_shape_job(lazy._h, ps.C.byref(job)) == 0
+    p, q = eager.problem, lazy.problem
+    F, nc = p.n_fac, ps.n_coef(p.lmax, p.mmax)
+    assert (job.nrows, job.lmax, job.mmax) == (6, p.lmax, p.mmax) and 8 * job.nrows ** 2 == F
+    assert (job.a, job.b, job.c) == (1.05 * eager.escl, 1.00 * eager.escl, 0.95 * eager.escl)
+    arr = lambda ptr, n: np.ctypeslib.as_array(ptr, shape=(n,))
+    assert ps.C.cast(job.dsph, ps.C.c_void_p).value == ps.C.cast(q.dsph, ps.C.c_void_p).value
+    assert not arr(q.dsph, F * nc).any() and not arr(q.cg_first, nc).any()
+    rc = ps.lib.ps_shape_basis(job.nrows, job.lmax, job.mmax, job.a, job.b, job.c, job.nor, job.darea, job.pleg,
+                               job.fc, job.fs, job.dsph, job.cg_first)
+    assert rc == F
+    L, M = p.lmax + 1, p.mmax + 1
+    for name, n in (("nor", 3 * F), ("darea", F), ("pleg", F * L * L), ("fc", F * M), ("fs", F * M),
+                    ("dsph", F * nc), ("cg_first", nc)):
+        assert np.array_equal(arr(getattr(q, name), n), arr(getattr(p, name), n)), name
+    # everything else is the same input
+    assert lazy.escl == eager.escl and lazy.n_periods == eager.n_periods
+    assert np.array_equal(arr(q.tim, p.n_points), arr(p.tim, p.n_points))
