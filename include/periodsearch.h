@@ -236,6 +236,18 @@ typedef struct ps_shape_job {
  * not touch the engine's problem.  Returns PS_OK or <0. */
 int ps_shape_basis_many(ps_engine *e, int n, const ps_shape_job *jobs);
 
+/* Post-scan selection over work-unit outputs (the project server's step
+ * after the app; not part of the reference repository): rows of one or more
+ * scans in one array.  *index = the row of least chi^2, the first of equal
+ * minima; rows whose chi^2 is NaN are skipped.  PS_ERR_ARG if no row has a
+ * number. */
+int ps_best_row(const ps_row *rows, long long n, long long *index);
+/* Read a period_search_out file (rows as ps_format_row writes them,
+ * period_search_BOINC.cpp:926-931) into rows[0..cap); *n = rows read.
+ * PS_ERR_IO on a malformed line, PS_ERR_LIMIT if the file has more than cap
+ * rows. */
+int ps_read_rows(const char *path, ps_row *rows, long long cap, long long *n);
+
 /* ps_input_read without the shape basis: the tables are sized and zero until
  * ps_shape_basis_many fills them through ps_input_shape_job's pointers. */
 int ps_input_read_deferred(const char *path, ps_input **out);

@@ -109,6 +109,8 @@ def _load() -> C.CDLL:
         "ps_input_problem": (C.c_int, [vp, C.POINTER(PsProblem)]),
         "ps_input_scalars": (C.c_int, [vp, _dp, _ip]),
         "ps_format_row": (C.c_int, [C.POINTER(PsRow), C.c_int, C.c_char_p, C.c_int]),
+        "ps_best_row": (C.c_int, [vp, C.c_longlong, C.POINTER(C.c_longlong)]),
+        "ps_read_rows": (C.c_int, [C.c_char_p, vp, C.c_longlong, C.POINTER(C.c_longlong)]),
         "ps_shape_basis": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "ps_lc_features": (C.c_int, [vp, _dp, C.c_int, C.c_int, _fp]),
@@ -138,7 +140,7 @@ EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_
             "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
             "ps_input_scalars", "ps_format_row", "ps_shape_basis", "ps_shape_basis_many", "ps_input_read_deferred",
-            "ps_input_shape_job",
+            "ps_input_shape_job", "ps_best_row", "ps_read_rows",
             "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
             "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
             "ps_mlp_train_step")
@@ -188,6 +190,37 @@ def _shape_arrays(nrows: int, lmax: int, mmax: int) -> dict:
     return dict(nor=np.zeros(F * 3), darea=np.zeros(F), pleg=np.zeros(F * (lmax + 1) ** 2),
                 fc=np.zeros(F * (mmax + 1)), fs=np.zeros(F * (mmax + 1)), dsph=np.zeros(F * n_coef(lmax, mmax)),
                 cg_first=np.zeros(n_coef(lmax, mmax)))
+
+
+def read_rows(path: str, cap: int = 1 << 20) -> np.ndarray:
+    """Rows of a period_search_out file (ps_read_rows)."""
+    rows = np.zeros(cap, dtype=ROW_DTYPE)
+    n = C.c_longlong()
+    rc = lib.ps_read_rows(os.fsencode(path), rows.ctypes.data, cap, C.byref(n))
+    if rc != 0:
+        raise PsError(f"ps_read_rows({path}) failed: {rc} after {n.value} rows")
+    return rows[:n.value].copy()
+
+
+def best_row(rows: np.ndarray) -> int:
+    """Index of the least-chi^2 row (ps_best_row): the period selection over
+    one or more scans' rows."""
+    rows = np.ascontiguousarray(rows, dtype=ROW_DTYPE)
+    idx = C.c_longlong()
+    rc = lib.ps_best_row(rows.ctypes.data, len(rows), C.byref(idx))
+    if rc != 0:
+        raise PsError(f"ps_best_row failed: {rc}")
+    return idx.value
+
+
+def best_period(paths) -> tuple:
+    """(path, row) of the least-chi^2 row over several period_search_out
+    files, e.g. the work units of one asteroid."""
+    parts = [read_rows(p) for p in paths]
+    owner = np.concatenate([np.full(len(r), k) for k, r in enumerate(parts)])
+    rows = np.concatenate(parts)
+    i = best_row(rows)
+    return paths[int(owner[i])], rows[i]
 
 
 class Input:

@@ -579,6 +579,51 @@ int ps_input_scalars(const ps_input *in, double *o, int *ndata) {
   return PS_OK;
 }
 
+int ps_best_row(const ps_row *rows, long long n, long long *index) {
+  if (!rows || !index || n < 1) return PS_ERR_ARG;
+  long long best = -1;
+  for (long long i = 0; i < n; i++) {
+    if (std::isnan(rows[i].chisq)) continue;
+    if (best < 0 || rows[i].chisq < rows[best].chisq) best = i;
+  }
+  if (best < 0) return PS_ERR_ARG;
+  *index = best;
+  return PS_OK;
+}
+
+int ps_read_rows(const char *path, ps_row *rows, long long cap, long long *n) {
+  if (!path || !n || (cap > 0 && !rows)) return PS_ERR_ARG;
+  *n = 0;
+  FILE *f = fopen(path, "r");
+  if (!f) return PS_ERR_IO;
+  char line[512];
+  long long k = 0;
+  int rc = PS_OK;
+  while (fgets(line, sizeof line, f)) {
+    ps_row r;
+    char extra;
+    int got = sscanf(line, "%lf %lf %lf %lf %lf %lf %c", &r.period_h, &r.rms, &r.chisq, &r.dark, &r.lambda,
+                     &r.beta, &extra);
+    if (got == EOF || got <= 0) {  // blank line
+      bool blank = true;
+      for (const char *c = line; *c; c++) blank = blank && (*c == ' ' || *c == '\t' || *c == '\n' || *c == '\r');
+      if (blank) continue;
+    }
+    if (got != 6) {
+      rc = PS_ERR_IO;
+      break;
+    }
+    if (k >= cap) {
+      rc = PS_ERR_LIMIT;
+      break;
+    }
+    rows[k++] = r;
+  }
+  fclose(f);
+  *n = k;
+  return rc;
+}
+
 int ps_format_row(const ps_row *r, int round_pole, char *buf, int buflen) {
   if (!r || !buf) return PS_ERR_ARG;
   double la = round_pole ? std::round(r->lambda) : r->lambda;

@@ -3,9 +3,12 @@ shape basis, checked against the oracle (no GPU needed)."""
 import re
 
 import numpy as np
+import pytest
 
 import oracle
 import periodsearch_amd as ps
+
+ROW_FIELDS = ("period_h", "rms", "chisq", "dark", "lambda", "beta")
 
 
 def test_every_declared_symbol_is_exported():
@@ -86,3 +89,44 @@ def test_deferred_input_hands_its_tables_to_the_shape_job(small_input):
     # everything else is the same input
     assert lazy.escl == eager.escl and lazy.n_periods == eager.n_periods
     assert np.array_equal(arr(q.tim, p.n_points), arr(p.tim, p.n_points))
+
+
+def test_read_rows_and_best_row_over_work_units(golden_dir, tmp_path):
+    """Post-scan selection: the reference-held sample output parses to its
+    134 rows, and the least-chi^2 row over several outputs is the one numpy
+    finds; the first of equal minima wins, NaN rows are skipped."""
+    import os
+
+    sample = os.path.join(golden_dir, "ps_sample_out")
+    rows = ps.read_rows(sample)
+    ref = np.loadtxt(sample)
+    assert len(rows) == len(ref) == 134
+    for k, name in enumerate(ROW_FIELDS):
+        assert np.array_equal(rows[name], ref[:, k]), name
+    assert ps.best_row(rows) == int(np.argmin(ref[:, 2]))
+    # split into three "work units" and select over them
+    paths = []
+    for k, (a, b) in enumerate(((0, 40), (40, 90), (90, 134))):
+        p = str(tmp_path / f"wu{k}")
+        with open(p, "w") as f:
+            for r in rows[a:b]:
+                f.write(ps.format_row(r))
+        paths.append(p)
+    path, row = ps.best_period(paths)
+    i = int(np.argmin(ref[:, 2]))
+    assert path == paths[0 if i < 40 else 1 if i < 90 else 2] and row["chisq"] == ref[i, 2]
+    # ties: first; NaN: skipped; nothing finite: error
+    t = rows[:5].copy()
+    t["chisq"] = [3.0, np.nan, 1.0, 1.0, 2.0]
+    assert ps.best_row(t) == 2
+    t["chisq"] = np.nan
+    with pytest.raises(ps.PsError):
+        ps.best_row(t)
+    # malformed line and capacity
+    bad = str(tmp_path / "bad")
+    with open(bad, "w") as f:
+        f.write("1.0 2.0 3.0 0.1 10 20\n1.0 2.0 x\n")
+    with pytest.raises(ps.PsError):
+        ps.read_rows(bad)
+    with pytest.raises(ps.PsError):
+        ps.read_rows(sample, cap=10)
