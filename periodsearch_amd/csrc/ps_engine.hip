@@ -1221,9 +1221,13 @@ __host__ __device__ inline MixedLds mixed_layout(int nfac, int nb, int mode) {
   return L;
 }
 
-__device__ inline unsigned bf16_bits(float v) {  // round to nearest even
-  const unsigned u = __float_as_uint(v);
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+// (a, b) rounded to bf16 (nearest even) and packed, a in the low half: one
+// v_cvt_pk_bf16_f32.
+__device__ inline unsigned bf16_pair(float a, float b) {
+  const v2bf h = __builtin_convertvector(v2f{a, b}, v2bf);
+  return __builtin_bit_cast(unsigned, h);
 }
 
 template <int NB, int MODE, bool MULTI>
@@ -1277,28 +1281,57 @@ __global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P0, co
   const T *__restrict__ dsT = BF ? reinterpret_cast<const T *>(P.dsphT16) : reinterpret_cast<const T *>(P.dsphT32);
   const T *__restrict__ dsTl = reinterpret_cast<const T *>(P.dsphT16lo);
   const int fpad = P.fpad;
-  for (int f0 = 0; f0 < F; f0 += kFacetChunk) {
-    // stage the chunk's Dsph^T rows (fpad is a multiple of the chunk)
-    for (int idx = tid; idx < NC * (kFacetChunk / PER16); idx += kThreads) {
-      const int c = idx / (kFacetChunk / PER16), pc = idx - c * (kFacetChunk / PER16);
-      *reinterpret_cast<uint4 *>(dch + c * WS + pc * PER16) =
-          *reinterpret_cast<const uint4 *>(dsT + (size_t)c * fpad + f0 + pc * PER16);
-      if (SPLIT)
-        *reinterpret_cast<uint4 *>(dchl + c * WS + pc * PER16) =
-            *reinterpret_cast<const uint4 *>(dsTl + (size_t)c * fpad + f0 + pc * PER16);
+  // Dsph^T chunks pass through registers one chunk ahead: the loads of chunk
+  // k+1 are in flight during chunk k's facet stage and contraction (fpad is
+  // a multiple of the chunk, so a chunk never reads past the table)
+  constexpr int kPieces = kFacetChunk / PER16;  // 16-byte pieces per row
+  constexpr int kStageIt = (NC * kPieces + kThreads - 1) / kThreads;
+  uint4 pre[kStageIt], prel[kStageIt];
+  auto load_chunk = [&](int fc) {
+#pragma unroll
+    for (int it = 0; it < kStageIt; it++) {
+      const int idx = tid + it * kThreads;
+      if (idx < NC * kPieces) {
+        const int c = idx / kPieces, pc = idx - c * kPieces;
+        pre[it] = *reinterpret_cast<const uint4 *>(dsT + (size_t)c * fpad + fc + pc * PER16);
+        if (SPLIT) prel[it] = *reinterpret_cast<const uint4 *>(dsTl + (size_t)c * fpad + fc + pc * PER16);
+      }
     }
-    // facet stage (FP32)
+  };
+  load_chunk(0);
+  for (int f0 = 0; f0 < F; f0 += kFacetChunk) {
+#pragma unroll
+    for (int it = 0; it < kStageIt; it++) {
+      const int idx = tid + it * kThreads;
+      if (idx < NC * kPieces) {
+        const int c = idx / kPieces, pc = idx - c * kPieces;
+        *reinterpret_cast<uint4 *>(dch + c * WS + pc * PER16) = pre[it];
+        if (SPLIT) *reinterpret_cast<uint4 *>(dchl + c * WS + pc * PER16) = prel[it];
+      }
+    }
+    if (f0 + kFacetChunk < F) load_chunk(f0 + kFacetChunk);
+    // facet stage (FP32).  The eight facets of a wave are wave-uniform: their
+    // normals (scalar loads) and areas are fetched before the first use.
+    const int fb = __builtin_amdgcn_readfirstlane(f0 + 8 * sub);
+    float4 nvs[8];
+    float afs[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int f = min(fb + i, F - 1);
+      nvs[i] = n4[f];
+      afs[i] = areaf[f];
+    }
     float wv[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const int f = f0 + 8 * sub + i;
+      const int f = fb + i;
       float w = 0.0f;
       if (f < F) {
-        const float4 nv = n4[f];
+        const float4 nv = nvs[i];
         const float mu = e_0 * nv.x + e_1 * nv.y + e_2 * nv.z;
         const float mu0 = s_0 * nv.x + s_1 * nv.y + s_2 * nv.z;
         if (mu > tiny && mu0 > tiny) {
-          const float af = areaf[f];
+          const float af = afs[i];
           const float rd = __builtin_amdgcn_rcpf(mu + mu0);
           const float mm = mu * mu0;
           w = af * (mm * (cl + cls * rd));
@@ -1320,17 +1353,18 @@ __global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P0, co
       wv[i] = w;
     }
     if constexpr (BF) {
-      unsigned h[8];
+      // v_cvt_pk_bf16_f32: two weights per instruction, round to nearest even
+      unsigned h[4];
 #pragma unroll
-      for (int i = 0; i < 8; i++) h[i] = bf16_bits(wv[i]);
-      *reinterpret_cast<uint4 *>(wt + pl * WS + 8 * sub) =
-          make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+      for (int q = 0; q < 4; q++) h[q] = bf16_pair(wv[2 * q], wv[2 * q + 1]);
+      *reinterpret_cast<uint4 *>(wt + pl * WS + 8 * sub) = make_uint4(h[0], h[1], h[2], h[3]);
       if constexpr (SPLIT) {
-        unsigned lo[8];
+        unsigned lo[4];
 #pragma unroll
-        for (int i = 0; i < 8; i++) lo[i] = bf16_bits(wv[i] - __uint_as_float(h[i] << 16));
-        *reinterpret_cast<uint4 *>(wtl + pl * WS + 8 * sub) = make_uint4(
-            lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16));
+        for (int q = 0; q < 4; q++)
+          lo[q] = bf16_pair(wv[2 * q] - __uint_as_float(h[q] << 16),
+                            wv[2 * q + 1] - __uint_as_float(h[q] & 0xffff0000u));
+        *reinterpret_cast<uint4 *>(wtl + pl * WS + 8 * sub) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
       }
     } else {
       *reinterpret_cast<float4 *>(wt + pl * WS + 8 * sub) = make_float4(wv[0], wv[1], wv[2], wv[3]);
