@@ -59,6 +59,18 @@ constexpr double kPi = 3.14159265358979323846;
 typedef double v4d __attribute__((ext_vector_type(4)));  // one 16x16 FP64 MFMA accumulator per lane
 typedef float v4f __attribute__((ext_vector_type(4)));   // one 16x16 FP32-accumulate MFMA accumulator per lane
 typedef __bf16 v8bf __attribute__((ext_vector_type(8))); // bf16 operand of v_mfma_f32_16x16x32_bf16
+typedef unsigned v4u __attribute__((ext_vector_type(4)));  // one 16-byte piece held in registers
+
+// f(std::integral_constant<int, I>) for I = B..N-1: a loop whose index is a
+// compile-time constant in the body, so that arrays indexed by it stay in
+// registers.
+template <int B, int N, class Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+  if constexpr (B < N) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, N>(f);
+  }
+}
 
 #define HIP_TRY(x)                                                         \
   do {                                                                     \
@@ -87,6 +99,7 @@ struct DevProblem {
   const unsigned short *dsphT16, *dsphT16lo;  // bf16 of Dsph, and of Dsph - hi (split form)
   int fpad;
   const double4 *nor4;    // [F] (nx, ny, nz, 0): one 32-byte scalar load per facet
+  const double *dsphP;    // [fpad][16*nb] Dsph zero-padded in rows and columns (FP64 matrix-core kernel)
   const double *cpleg;    // [ngrp][F]: P_l^m of coefficient group q at facet f
   const double *cfc;      // [mmax+1][F]
   const double *cfs;      // [mmax+1][F]
@@ -1034,7 +1047,7 @@ __global__ void __launch_bounds__(kThreads, PS_MODEL_MIN_WAVES) model_kernel(Dev
 //      Dsph chunk[kFacetChunk][16 NB]; the last two regions are reused for
 //      the partial-sum reduction at the end.
 template <int NB, bool MULTI>
-__global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+__global__ void __launch_bounds__(kThreads, NB <= 4 ? 3 : 2) model_mfma_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
                                                               DevWork W) {
   extern __shared__ double sm[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
@@ -1074,25 +1087,54 @@ __global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P0, con
 #pragma unroll
   for (int cb = 0; cb < NB; cb++) acc[cb] = v4d{0, 0, 0, 0};
   const double4 *__restrict__ n4 = P.nor4;
-  const double *__restrict__ dsp = P.dsph;
-  const int ncp = P.ncp;
+  // Dsph chunks ([fpad][NC], zero-padded: a chunk is 32 contiguous rows) pass
+  // through registers one chunk ahead, so their loads are in flight during
+  // the previous chunk's facet stage and contraction
+  const v4u *__restrict__ dsq = reinterpret_cast<const v4u *>(P.dsphP);
+  constexpr int kPieces = kFacetChunk * NC / 2;  // 16-byte pieces per chunk
+  constexpr int kStageIt = (kPieces + kThreads - 1) / kThreads;
+  v4u pre[kStageIt];
+  // (threads past the last piece repeat it: identical loads and stores, no
+  // branches, so the pieces stay in registers)
+  auto load_chunk = [&](int fc) __attribute__((always_inline)) {
+    static_for<0, kStageIt>([&](auto c) __attribute__((always_inline)) {
+      constexpr int it = decltype(c)::value;
+      const int idx = min(tid + it * kThreads, kPieces - 1);
+      pre[it] = dsq[(size_t)fc * (NC / 2) + idx];
+    });
+  };
+  load_chunk(0);
   for (int f0 = 0; f0 < F; f0 += kFacetChunk) {
-    // stage the chunk's Dsph rows (zero beyond F and beyond ncp)
-    for (int idx = tid; idx < kFacetChunk * NC; idx += kThreads) {
-      const int r = idx / NC, c = idx - r * NC, f = f0 + r;
-      dch[idx] = (f < F && c < ncp) ? dsp[(size_t)f * ncp + c] : 0.0;
-    }
-    // facet stage
-#pragma unroll 2
-    for (int i = 0; i < kFacetChunk / 4; i++) {
-      const int fl = sub + 4 * i, f = f0 + fl;
+    static_for<0, kStageIt>([&](auto c) __attribute__((always_inline)) {
+      constexpr int it = decltype(c)::value;
+      const int idx = min(tid + it * kThreads, kPieces - 1);
+      reinterpret_cast<v4u *>(dch)[idx] = pre[it];
+    });
+    if (f0 + kFacetChunk < F) load_chunk(f0 + kFacetChunk);
+    // facet stage; a wave's facets are wave-uniform: normals by scalar
+    // loads and areas fetched four facets at a time, before their use
+    const int fb = __builtin_amdgcn_readfirstlane(f0 + sub);
+#pragma unroll
+    for (int g = 0; g < kFacetChunk / 16; g++) {
+      double4 nvs[4];
+      double afs[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int f = min(fb + 4 * (4 * g + j), F - 1);
+        nvs[j] = n4[f];
+        afs[j] = area[f];
+      }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = 4 * g + j;
+      const int fl = sub + 4 * i, f = fb + 4 * i;
       double wv = 0.0;
       if (f < F) {
-        const double4 nv = n4[f];
+        const double4 nv = nvs[j];
         const double mu = e_0 * nv.x + e_1 * nv.y + e_2 * nv.z;
         const double mu0 = s_0 * nv.x + s_1 * nv.y + s_2 * nv.z;
         if (mu > kTiny && mu0 > kTiny) {
-          const double af = area[f];
+          const double af = afs[j];
           const double rd = recip(mu + mu0);
           const double mm = mu * mu0;
           wv = af * (mm * (cl + cls * rd));
@@ -1112,6 +1154,7 @@ __global__ void __launch_bounds__(kThreads) model_mfma_kernel(DevProblem P0, con
         }
       }
       wt[fl * kTilePts + pl] = wv;
+    }
     }
     __syncthreads();
     // contraction on the matrix cores: wave = row block of 16 points
@@ -1286,29 +1329,27 @@ __global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P0, co
   // a multiple of the chunk, so a chunk never reads past the table)
   constexpr int kPieces = kFacetChunk / PER16;  // 16-byte pieces per row
   constexpr int kStageIt = (NC * kPieces + kThreads - 1) / kThreads;
-  uint4 pre[kStageIt], prel[kStageIt];
-  auto load_chunk = [&](int fc) {
-#pragma unroll
-    for (int it = 0; it < kStageIt; it++) {
-      const int idx = tid + it * kThreads;
-      if (idx < NC * kPieces) {
-        const int c = idx / kPieces, pc = idx - c * kPieces;
-        pre[it] = *reinterpret_cast<const uint4 *>(dsT + (size_t)c * fpad + fc + pc * PER16);
-        if (SPLIT) prel[it] = *reinterpret_cast<const uint4 *>(dsTl + (size_t)c * fpad + fc + pc * PER16);
-      }
-    }
+  v4u pre[kStageIt], prel[kStageIt];
+  // (threads past the last piece repeat it: identical loads and stores, no
+  // branches, so the pieces stay in registers)
+  auto load_chunk = [&](int fc) __attribute__((always_inline)) {
+    static_for<0, kStageIt>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int it = decltype(ic)::value;
+      const int idx = min(tid + it * kThreads, NC * kPieces - 1);
+      const int c = idx / kPieces, pc = idx - c * kPieces;
+      pre[it] = *reinterpret_cast<const v4u *>(dsT + (size_t)c * fpad + fc + pc * PER16);
+      if (SPLIT) prel[it] = *reinterpret_cast<const v4u *>(dsTl + (size_t)c * fpad + fc + pc * PER16);
+    });
   };
   load_chunk(0);
   for (int f0 = 0; f0 < F; f0 += kFacetChunk) {
-#pragma unroll
-    for (int it = 0; it < kStageIt; it++) {
-      const int idx = tid + it * kThreads;
-      if (idx < NC * kPieces) {
-        const int c = idx / kPieces, pc = idx - c * kPieces;
-        *reinterpret_cast<uint4 *>(dch + c * WS + pc * PER16) = pre[it];
-        if (SPLIT) *reinterpret_cast<uint4 *>(dchl + c * WS + pc * PER16) = prel[it];
-      }
-    }
+    static_for<0, kStageIt>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int it = decltype(ic)::value;
+      const int idx = min(tid + it * kThreads, NC * kPieces - 1);
+      const int c = idx / kPieces, pc = idx - c * kPieces;
+      *reinterpret_cast<v4u *>(dch + c * WS + pc * PER16) = pre[it];
+      if (SPLIT) *reinterpret_cast<v4u *>(dchl + c * WS + pc * PER16) = prel[it];
+    });
     if (f0 + kFacetChunk < F) load_chunk(f0 + kFacetChunk);
     // facet stage (FP32).  The eight facets of a wave are wave-uniform: their
     // normals (scalar loads) and areas are fetched before the first use.
@@ -2103,6 +2144,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk, ncT = 16 * nb;
   std::vector<float> dsphT32((size_t)ncT * fpad, 0.0f);
   std::vector<unsigned short> dsphT16((size_t)ncT * fpad, 0), dsphT16lo((size_t)ncT * fpad, 0);
+  std::vector<double> dsphP((size_t)fpad * ncT, 0.0);
   auto to_bf16 = [](float v) {
     unsigned u;
     memcpy(&u, &v, 4);
@@ -2115,6 +2157,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
     for (int c = 0; c < ncoef; c++) {
       const double v = p->dsph[(size_t)f * ncoef + c];
       dsph[(size_t)f * ncp + c] = v;
+      dsphP[(size_t)f * ncT + c] = v;
       const float vf = (float)v;
       dsphT32[(size_t)c * fpad + f] = vf;
       const unsigned short hi = to_bf16(vf);
@@ -2160,6 +2203,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   UP(nz.data(), F, nz);
   UP(p->darea, F, darea);
   UP(dsph.data(), dsph.size(), dsph);
+  UP(dsphP.data(), dsphP.size(), dsphP);
   UP(nor4.data(), F, nor4);
   UP(nor4f.data(), F, nor4f);
   UP(dsphT32.data(), dsphT32.size(), dsphT32);
