@@ -508,7 +508,7 @@ __host__ __device__ inline int neq_jn_doubles(int mfit, bool mix) {
 }
 
 template <int NT, bool MIX>
-__device__ void normal_equations_mfma(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
+__device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
                                       double *drat, double *beta, double *chi_out, double *alpha) {
 #pragma clang fp contract(off)
   using AccT = typename std::conditional<MIX, v4f, v4d>::type;
@@ -741,7 +741,7 @@ __device__ inline void pivot_max(double &bv, int &be, double ov, int oe) {
   }
 }
 
-__device__ int gauss_solve(double *buf, int n, double *x) {
+__device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
 #pragma clang fp contract(off)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ld = n + 1, nn = n * ld;
@@ -841,10 +841,16 @@ __device__ int gauss_solve(double *buf, int n, double *x) {
 // the device array of problems.
 template <bool MULTI>
 __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, const DevProblem *probs, int idx) {
-  if constexpr (MULTI)
-    return probs[__builtin_amdgcn_readfirstlane(idx)];
-  else
+  if constexpr (MULTI) {
+    // The problem array is read-only for the whole kernel: reading it through
+    // the constant address space lets its (wave-uniform) fields come in by
+    // scalar loads even after the kernel's own global stores.
+    typedef const __attribute__((address_space(4))) DevProblem ConstProblem;
+    ConstProblem *cp = (ConstProblem *)probs + __builtin_amdgcn_readfirstlane(idx);
+    return *(const DevProblem *)cp;
+  } else {
     return P0;
+  }
 }
 
 // step_kernel: one workgroup per slot.  Finishes a fit whose loop predicate
