@@ -177,8 +177,8 @@ def main() -> int:
     ap.add_argument("--window", type=int, default=16, help="batch64: trial periods per asteroid per step")
     ap.add_argument("--autotune", action="store_true",
                     help="time slot counts and slot groups on this workload first and keep the fastest")
-    ap.add_argument("--candidates", type=int, default=4, help="learned form: periods verified per block")
-    ap.add_argument("--block", type=int, default=1024, help="learned form: trial periods per block")
+    ap.add_argument("--candidates", type=int, default=16, help="learned form: periods verified per block")
+    ap.add_argument("--block", type=int, default=4096, help="learned form: trial periods per block")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -208,15 +208,24 @@ def main() -> int:
     inp = Input(path)
     eng = Engine(0)
     eng.set_precision(prec)
+    # learned form: one batch per block holds the block's proposed fits and the
+    # previous block's verification fits (candidates x 5 periods x 10 poles)
+    learned_batch = args.block + 10 * args.candidates * 5
     if args.slots:
         eng.set_slots(args.slots)
+    elif args.form == "learned":
+        eng.set_slots(learned_batch)
     eng.set_problem(inp.problem, keep=inp)
     tuned = None
     if args.autotune:
         from periodsearch_amd.autotune import autotune
 
         # periods far from the timed ones (the same per-fit cost under stop 50)
-        tuned = autotune(eng, inp, inp.conw_r0, n_from=inp.n_periods // 2)
+        if args.form == "learned":
+            tuned = autotune(eng, inp, inp.conw_r0, n_fits=learned_batch, slots=(2048, 3072, learned_batch),
+                             n_from=inp.n_periods // 2)
+        else:
+            tuned = autotune(eng, inp, inp.conw_r0, n_from=inp.n_periods // 2)
     nc, ma, mfit = eng.dims()
     N, F = inp.ndata, inp.problem.n_fac
     conw_r = inp.conw_r0
@@ -293,6 +302,8 @@ def main() -> int:
     }
     if tuned:
         out["config"]["autotuned"] = {"slots": tuned["best"]["slots"], "groups": tuned["best"]["groups"]}
+    else:
+        out["config"]["slots"] = args.slots or (learned_batch if args.form == "learned" else 2048)
     if args.form == "learned":
         out["config"]["learned"] = (f"1 proposed start per trial period; in each {args.block}-period block the "
                                     f"{args.candidates} best periods and 2 neighbours each refitted from the 10 "

@@ -123,3 +123,41 @@ def test_learned_scan_keeps_the_plain_best_period_with_fewer_fits():
     assert ev["chi2_rel_max"] < 1e-12
     assert ev["fit_ratio"] > 2.0
     prop.close()
+
+
+def test_learned_scan_in_bench_blocks_keeps_the_plain_best_period(tmp_path):
+    """The bench's learned configuration on the bench asteroid (1080 points,
+    lmax 6, 512 facets): one block of 4096 trial periods around the true
+    period, its 16 best periods and 2 neighbours each verified with the ten
+    plain poles.  The best verified period is the plain scan's best period
+    of the same 4096 periods, with the same chi2 (the verified rows are the
+    plain fits themselves), from 8x fewer LM fits."""
+    import numpy as np
+
+    from periodsearch_amd import PS_PREC_BF16X3, Engine, Input, synth
+    from periodsearch_amd import proposal as pr
+
+    path = str(tmp_path / "period_search_in")
+    synth.write_workload("wu", path)
+    inp = Input(path)
+    eng = Engine(0)
+    eng.set_precision(PS_PREC_BF16X3)
+    eng.set_slots(4096 + 800)
+    eng.set_problem(inp.problem, keep=inp)
+    P = synth.WORKLOADS["wu"]["period_h"]
+    n_true = int(round((inp.freq_start - 1.0 / P) / inp.freq_step)) + 1
+    n0 = max(1, n_true - 2048)
+    n1 = n0 + 4095
+    plain = eng.scan(inp.freq_start, inp.freq_step, n0, n1, inp.conw_r0)
+    prop = pr.Proposal.load()
+    rows, ver, nf, _ = pr.scan_learned(eng, inp, prop, inp.conw_r0, n0, n1, n_candidates=16, halo=2, block=4096)
+    kp = int(np.argmin(plain["chisq"]))
+    kl = int(np.flatnonzero(ver)[np.argmin(rows["chisq"][ver])])
+    step_h = P * P * inp.freq_step
+    print(f"plain best row {kp} P {plain['period_h'][kp]:.6f} h chi2 {plain['chisq'][kp]:.8g}; learned best row {kl} "
+          f"P {rows['period_h'][kl]:.6f} h chi2 {rows['chisq'][kl]:.8g}; LM fits {10 * 4096} / {nf}")
+    assert abs(rows["period_h"][kl] - plain["period_h"][kp]) <= step_h
+    assert abs(rows["chisq"][kl] - plain["chisq"][kp]) <= 1e-12 * plain["chisq"][kp]
+    assert 10 * 4096 / nf > 7.5
+    prop.close()
+    eng.close()
