@@ -509,16 +509,11 @@ __host__ __device__ inline int neq_jn_doubles(int mfit, bool mix) {
 
 template <int NT, bool MIX>
 __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
-                                      double *drat, double *beta, double *chi_out, double *alpha) {
+                                      double *drat, const int *fidx, double *beta, double *chi_out, double *alpha) {
 #pragma clang fp contract(off)
   using AccT = typename std::conditional<MIX, v4f, v4d>::type;
   using JnT = typename std::conditional<MIX, float, double>::type;
   const float *Jf = reinterpret_cast<const float *>(J);
-  // entry (l, q) of the Jacobian, l <= ma (row ma = ymod is FP64 in both forms)
-  auto jv = [&](int l, int q) -> double {
-    if (MIX && l < P.ma) return (double)Jf[(size_t)l * P.npad + q];
-    return J[(size_t)l * P.npad + q];
-  };
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ld = P.npad, mfit = P.mfit, ma = P.ma;
   const int nb = (mfit + 15) / 16, MP = 16 * nb, MS = MP + 1;
@@ -612,13 +607,26 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
           aux[pp * 3 + 2] = 0;
         }
         const double ave = dave[ma];
-        // eight rows per batch: their loads are issued together
+        // eight rows per batch: their loads are issued together (the row
+        // indices come from LDS, so no load waits on another)
         for (int jf0 = wave; jf0 < MP; jf0 += 8 * kWaves) {
           double dv[8];
+          int lv[8];
+          JnT raw[8];
+          const int qc = live ? q : q0;  // loads without branches: a valid point for idle lanes
 #pragma unroll
           for (int u = 0; u < 8; u++) {
             const int jf = jf0 + kWaves * u;
-            dv[u] = (jf < mfit && live) ? jv(P.fidx[jf], q) : 0.0;
+            lv[u] = jf < mfit ? fidx[jf] : 0;
+            if constexpr (MIX)
+              raw[u] = Jf[(size_t)lv[u] * ld + qc];
+            else
+              raw[u] = J[(size_t)lv[u] * ld + qc];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int jf = jf0 + kWaves * u;
+            dv[u] = (jf < mfit && live) ? (double)raw[u] : 0.0;
           }
 #pragma unroll
           for (int u = 0; u < 8; u++) {
@@ -626,7 +634,7 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
             if (jf >= MP) break;
             double d = dv[u];
             if (inrel && jf < mfit && live) {
-              const int l = P.fidx[jf];
+              const int l = lv[u];
               if (MIX)
                 d = (l == 0) ? 0.0 : coef * (d - ymr * drat[l]);
               else
@@ -1511,17 +1519,19 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevP
   double *wst = sm, *dave = wst + neq_jn_doubles(mfit, MIX) + 3 * kNeqChunk;
   double *x = dave + kMaxMa + 1, *scal = x + kMaxMa;  // scal[0] = chi2, [1..3] = chck
   double *drat = scal + 4;                            // [ma] dave/ave (mixed forms)
+  int *fidx = reinterpret_cast<int *>(drat + kMaxMa); // [mfit] free parameter -> index into a[]
   const int eval = S.eval;
   const int sel = S.sel;
   double *atr = mem + (sel ? M.alpha0 : M.alpha1), *btr = mem + (sel ? M.beta0 : M.beta1);
   if (eval) {
     if (tid == 0) atomicAdd(W.done + 1, 1);
     for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
+    for (int jf = tid; jf < mfit; jf += kThreads) fidx[jf] = P.fidx[jf];
     conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1, wst);
     __threadfence_block();
     __syncthreads();
-    normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, btr,
-                                   scal, atr);
+    normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx,
+                                   btr, scal, atr);
     __threadfence_block();
     __syncthreads();
   }
@@ -1766,7 +1776,8 @@ size_t model_lds(int nfac) {
   return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
 }
 size_t neq_lds(int mfit, bool mix) {
-  return ((size_t)neq_jn_doubles(mfit, mix) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8) * sizeof(double);
+  return ((size_t)neq_jn_doubles(mfit, mix) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8 + (kMaxMfit + 1) / 2) *
+         sizeof(double);
 }
 
 using NeqFn = void (*)(DevProblem, const DevProblem *, DevWork);
