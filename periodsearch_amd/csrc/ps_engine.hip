@@ -976,10 +976,26 @@ __global__ void __launch_bounds__(kStepThreads, 4) step_kernel(DevProblem P0, co
   {
 #pragma clang fp contract(off)
     const double lam = S.lam;
-    const int ld = mfit + 1;
-    for (int j = tid >> 6; j < mfit; j += kWaves) {
-      for (int k = tid & 63; k <= mfit; k += 64)
-        A[j * ld + k] = (k == mfit) ? bacc[j] : ((j == k) ? aacc[j * mfit + k] * (1 + lam) : aacc[j * mfit + k]);
+    const int ld = mfit + 1, wave = tid >> 6, lane = tid & 63;
+    // four rows at a time, loads without branches (clamped indices) so that
+    // they are in flight together
+    for (int j0 = wave; j0 < mfit; j0 += 4 * kWaves) {
+      for (int k = lane; k <= mfit; k += 64) {
+        const int kc = min(k, mfit - 1);
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int j = min(j0 + kWaves * q, mfit - 1);
+          const double av = aacc[j * mfit + kc], bv = bacc[j];
+          const double damp = av * ((j == k) ? (1 + lam) : 1.0);  // times 1.0 is exact
+          v[q] = (k == mfit) ? bv : damp;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int j = j0 + kWaves * q;
+          if (j < mfit) A[j * ld + k] = v[q];
+        }
+      }
     }
   }
   __syncthreads();
