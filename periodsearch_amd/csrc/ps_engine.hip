@@ -536,36 +536,45 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
     const int inrel = P.lc_inrel[lc];
     if (inrel) {
       // light-curve sums (mrqcof.c:56-75): ave = sum ymod, dave = sum dyda.
-      // Each column is summed by 4 threads over interleaved quarters of the
-      // points (8 independent loads in flight each), then combined in order.
-      double *part = wst;  // [4][ma+1] scratch, free before the chunk loop
-      for (int cs = tid; cs < 4 * (ma + 1); cs += kThreads) {
-        const int col = cs % (ma + 1), seg = cs / (ma + 1);
-        double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int q = q0 + seg;
-        if (MIX && col < ma) {
-          const float *cp = Jf + (size_t)col * ld;
-          for (; q + 28 < q1; q += 32) {
+      // Wave w sums columns w, w+4, ..., two columns at a time with the lanes
+      // over the points (coalesced loads, four partial sums per lane and
+      // column, eight loads in flight), then a butterfly over the wave: at
+      // every step lanes l and l^o add the same two values, so every lane
+      // ends with the same total.
+      auto lc_sums = [&](auto *pa, auto *pb, bool two, int ca, int cb) __attribute__((always_inline)) {
+        double sa[4] = {0, 0, 0, 0}, sb[4] = {0, 0, 0, 0};
+        for (int qb = q0; qb < q1; qb += 256) {
 #pragma unroll
-            for (int u = 0; u < 8; u++) s8[u] = s8[u] + (double)cp[q + 4 * u];
+          for (int u = 0; u < 4; u++) {
+            const int q = qb + lane + 64 * u;
+            const bool in = q < q1;
+            const int qc = in ? q : q0;
+            const double va = (double)pa[qc], vb = (double)pb[qc];
+            sa[u] = sa[u] + (in ? va : 0.0);
+            sb[u] = sb[u] + (in ? vb : 0.0);
           }
-          for (int u = 0; q < q1; q += 4, u++) s8[u & 7] = s8[u & 7] + (double)cp[q];
-        } else {
-          const double *cp = J + (size_t)col * ld;
-          for (; q + 28 < q1; q += 32) {
-#pragma unroll
-            for (int u = 0; u < 8; u++) s8[u] = s8[u] + cp[q + 4 * u];
-          }
-          for (int u = 0; q < q1; q += 4, u++) s8[u & 7] = s8[u & 7] + cp[q];
         }
-        double sum = 0;
-#pragma unroll
-        for (int u = 0; u < 8; u++) sum = sum + s8[u];
-        part[seg * (ma + 1) + col] = sum;
+        double ta = (sa[0] + sa[1]) + (sa[2] + sa[3]), tb = (sb[0] + sb[1]) + (sb[2] + sb[3]);
+        for (int o = 32; o > 0; o >>= 1) {
+          ta = ta + __shfl_xor(ta, o, 64);
+          tb = tb + __shfl_xor(tb, o, 64);
+        }
+        if (lane == 0) {
+          dave[ca] = ta;
+          if (two) dave[cb] = tb;
+        }
+      };
+      for (int ca = wave; ca <= ma; ca += 2 * kWaves) {
+        const int cb0 = ca + kWaves;
+        const bool two = cb0 <= ma;
+        const int cb = two ? cb0 : ca;
+        if (MIX && cb < ma)  // both derivative rows (FP32)
+          lc_sums(Jf + (size_t)ca * ld, Jf + (size_t)cb * ld, two, ca, cb);
+        else if (MIX && ca < ma)  // ca FP32, cb the model row (FP64)
+          lc_sums(Jf + (size_t)ca * ld, J + (size_t)cb * ld, two, ca, cb);
+        else
+          lc_sums(J + (size_t)ca * ld, J + (size_t)cb * ld, two, ca, cb);
       }
-      __syncthreads();
-      for (int col = tid; col <= ma; col += kThreads)
-        dave[col] = ((part[col] + part[(ma + 1) + col]) + part[2 * (ma + 1) + col]) + part[3 * (ma + 1) + col];
       __syncthreads();
       if (MIX) {  // dave[l]/ave, once per light curve
         for (int col = tid; col < ma; col += kThreads) drat[col] = dave[col] / dave[ma];
