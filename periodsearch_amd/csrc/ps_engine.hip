@@ -1313,7 +1313,7 @@ __device__ inline unsigned bf16_pair(float a, float b) {
 }
 
 template <int NB, int MODE, bool MULTI>
-__global__ void __launch_bounds__(kThreads) model_mixed_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+__global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) model_mixed_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
                                                                DevWork W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
