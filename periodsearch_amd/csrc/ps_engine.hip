@@ -853,6 +853,115 @@ __device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
   return 0;
 }
 
+// Broadcast lane `lane` (wave-uniform) of a double through scalar registers.
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Pivot candidate (value, row-major index) reduced over the wave: a
+// butterfly in which lanes l and l^o combine the same two candidates, so
+// every lane ends with the same winner (larger value; equal values: larger
+// index, the reference's ">=" scan keeps the last).
+__device__ __forceinline__ void wave_pivot_max(double &bv, int &be) {
+  for (int o = 32; o > 0; o >>= 1) pivot_max(bv, be, __shfl_xor(bv, o, 64), __shfl_xor(be, o, 64));
+  be = __builtin_amdgcn_readfirstlane(be);
+}
+
+// gauss_errc (period_search/gauss_errc.c:9-73) by one wave with the
+// augmented matrix in registers: lane c holds column c of [A|b] (c <= n <=
+// R - 1, R <= 64), a[p] its entry in physical row p.  Rows never move: the
+// reference's swap of rows irow and icol becomes a swap in a logical-row
+// permutation (perm, lane L: physical row of logical row L; inv, lane p:
+// logical row of physical row p).  Pivot candidates and ties are judged by
+// the logical row-major index j*n + c, as the reference scans, and every
+// value b depends on is computed with the reference's operations:
+//   pr = a[icol][c] * pivinv; a[j][c] = a[j][c] - pr * a[j][icol].
+// Pivoted columns (lanes) keep being updated but are never read again.
+// No LDS and no barriers; returns 0, 1 (singular) or 2 (zero pivot), the
+// solution in x[0..n) (written by lane 0) on 0.
+template <int R>
+__device__ __forceinline__ int gauss_solve_wave(double (&a)[R], int n, double *x) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  int perm = lane, inv = lane;
+  unsigned long long done = 0ull;
+  const bool colc = lane < n;
+  double bv = -1.0;
+  int be = -1;
+#pragma unroll
+  for (int p = 0; p < R; p++) {
+    if (p < n && colc) {  // physical = logical order here: indices increase with p
+      const double v = fabs(a[p]);
+      if (v >= bv) {
+        bv = v;
+        be = p * n + lane;
+      }
+    }
+  }
+  wave_pivot_max(bv, be);
+  int prev_irow = -1, prev_icol = -1;
+  for (int i = 0; i < n; i++) {
+    int irow, icol;
+    if (be < 0) {  // no candidate (NaN matrix): the reference keeps the previous pivot
+      if (prev_icol < 0) return 1;
+      irow = prev_irow;
+      icol = prev_icol;
+    } else {
+      irow = be / n;
+      icol = be - irow * n;
+    }
+    if ((done >> icol) & 1ull) return 1;  // column pivoted twice: the next scan reports singular
+    prev_irow = irow;
+    prev_icol = icol;
+    const int phr = __builtin_amdgcn_readlane(perm, irow), phc = __builtin_amdgcn_readlane(perm, icol);
+    if (irow != icol) {
+      perm = (lane == irow) ? phc : ((lane == icol) ? phr : perm);
+      inv = (lane == phr) ? icol : ((lane == phc) ? irow : inv);
+    }
+    const int pp = phr;  // physical row of the pivot row (logical row icol after the swap)
+    double prow = 0.0;
+#pragma unroll
+    for (int p = 0; p < R; p++) prow = (p == pp) ? a[p] : prow;
+    const double piv = lane_value(prow, icol);
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    done |= 1ull << icol;
+    const double pr = prow * pivinv;  // new a[icol][c] (b for c == n)
+    const bool candc = colc && !((done >> lane) & 1ull);
+    bv = -1.0;
+    be = -1;
+#pragma unroll
+    for (int p = 0; p < R; p++) {
+      if (p < n) {
+        const double dum = lane_value(a[p], icol);
+        const double v = (p == pp) ? pr : a[p] - pr * dum;
+        a[p] = v;
+        const int j = __builtin_amdgcn_readlane(inv, p);
+        if (candc && !((done >> j) & 1ull)) {
+          const double av = fabs(v);
+          const int idx = j * n + lane;
+          if (av > bv || (av == bv && idx > be)) {
+            bv = av;
+            be = idx;
+          }
+        }
+      }
+    }
+    wave_pivot_max(bv, be);
+  }
+#pragma unroll
+  for (int p = 0; p < R; p++) {
+    if (p < n) {
+      const double bp = lane_value(a[p], n);
+      const int j = __builtin_amdgcn_readlane(inv, p);
+      if (lane == 0) x[j] = bp;
+    }
+  }
+  return 0;
+}
+
 // Problem of a slot.  One problem (MULTI false): the kernel argument P0, as
 // constant as any other argument.  Several (MULTI true): the slot's entry of
 // the device array of problems.
@@ -876,10 +985,14 @@ __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, 
 // parameters (their curv() runs in curv_kernel).
 // LDS: the solver's two augmented matrices and pivot exchange; the final
 // curv of a finished fit (area/g [F]) reuses the same space.
-constexpr int kStepThreads = kThreads;
-template <bool MULTI>
-__global__ void __launch_bounds__(kStepThreads, 4) step_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
-                                                          DevWork W) {
+// WAVE (mfit <= 63): one wave per slot, the solve in registers
+// (gauss_solve_wave); otherwise 256 threads and the LDS solver.
+constexpr int kWaveSolveMax = 63;
+template <bool MULTI, bool WAVE>
+__global__ void __launch_bounds__(WAVE ? 64 : kThreads, WAVE ? 1 : 4) step_kernel(DevProblem P0,
+                                                                            const DevProblem *__restrict__ probs,
+                                                                            DevWork W) {
+  constexpr int kStepThreads = WAVE ? 64 : kThreads;
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
   SlotState &S = W.st[slot];
@@ -982,6 +1095,25 @@ __global__ void __launch_bounds__(kStepThreads, 4) step_kernel(DevProblem P0, co
   const double *aacc = mem + (S.sel ? M.alpha1 : M.alpha0);
   const double *bacc = mem + (S.sel ? M.beta1 : M.beta0);
   double *da = mem + M.da;
+  int err;
+  if constexpr (WAVE) {
+#pragma clang fp contract(off)
+    // lane c loads column c of the damped [alpha | beta] (c <= mfit)
+    const double lam = S.lam;
+    const int lane = tid;
+    // lanes <= mfit read alpha column min(lane, mfit-1); lane mfit takes beta
+    const double *colp = (lane == mfit) ? bacc : aacc + min(lane, mfit - 1);
+    const int rstride = (lane == mfit) ? 1 : mfit;
+    double am[kWaveSolveMax + 1];
+#pragma unroll
+    for (int p = 0; p <= kWaveSolveMax; p++) {
+      const int j = min(p, mfit - 1);
+      const double av = colp[j * rstride];
+      am[p] = av * ((p == lane) ? (1 + lam) : 1.0);  // times 1.0 is exact; beta's lane never matches
+    }
+    err = gauss_solve_wave(am, mfit, da);
+    __syncthreads();
+  } else {
   {
 #pragma clang fp contract(off)
     const double lam = S.lam;
@@ -1008,8 +1140,9 @@ __global__ void __launch_bounds__(kStepThreads, 4) step_kernel(DevProblem P0, co
     }
   }
   __syncthreads();
-  const int err = gauss_solve(A, mfit, da);
+  err = gauss_solve(A, mfit, da);
   __syncthreads();
+  }
   double *a = mem + M.a, *atry = mem + M.atry;
   if (err == 0) {
 #pragma clang fp contract(off)
@@ -1793,8 +1926,8 @@ size_t mfma_lds(int nfac, int ncoef) {
   return d * sizeof(double) + kTilePts * sizeof(PointGeom);
 }
 
-size_t step_lds(int mfit, int nfac) {
-  const size_t solve = 2 * (size_t)mfit * (mfit + 1) + 4 * kWaves, fin = 2 * (size_t)nfac;
+size_t step_lds(int mfit, int nfac) {  // mfit = 0: the register solver (no LDS for the solve)
+  const size_t solve = mfit ? 2 * (size_t)mfit * (mfit + 1) + 4 * kWaves : 0, fin = 2 * (size_t)nfac;
   return (solve > fin ? solve : fin) * sizeof(double);
 }
 size_t model_lds(int nfac) {
@@ -1842,6 +1975,7 @@ struct ps_engine {
   float *d_feat = nullptr;
   size_t feat_cap = 0;
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
+  int step_threads = 256;  // one wave when the solve fits in registers
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
   void (*step)(DevProblem, const DevProblem *, DevWork) = nullptr;
@@ -2002,7 +2136,7 @@ int run_batch(ps_engine *e, DevWork W) {
       for (int g = 0; g < ng; g++) {
         const DevWork &Wk = Wg[g];
         const int ns = Wk.nslots;
-        hipLaunchKernelGGL(e->step, dim3(ns), dim3(kStepThreads), e->lds_step, sg[g], P0, probs, Wk);
+        hipLaunchKernelGGL(e->step, dim3(ns), dim3(e->step_threads), e->lds_step, sg[g], P0, probs, Wk);
         hipLaunchKernelGGL(e->curv, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g], P0,
                            probs, Wk);
         if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g)], sg[g]));
@@ -2461,7 +2595,8 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
   e->prob_npoles.resize(n);
   for (int i = 0; i < n; i++) e->prob_npoles[i] = p[i].n_poles;
   e->model = mk;
-  e->lds_step = step_lds(mfit, F);
+  const bool wave_solve = mfit <= kWaveSolveMax;
+  e->lds_step = step_lds(wave_solve ? 0 : mfit, F);
   e->lds_model = mixed ? mixed_layout(F, nb, mode).total : (e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
   e->lds_neq = neq_lds(mfit, mixed);
   const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
@@ -2470,7 +2605,9 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;
   }
-  e->step = multi ? step_kernel<true> : step_kernel<false>;
+  e->step = wave_solve ? (multi ? step_kernel<true, true> : step_kernel<false, true>)
+                       : (multi ? step_kernel<true, false> : step_kernel<false, false>);
+  e->step_threads = wave_solve ? 64 : kThreads;
   e->curv = multi ? curv_kernel<true> : curv_kernel<false>;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(e->step), hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)e->lds_step));
