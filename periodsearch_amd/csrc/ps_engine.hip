@@ -752,10 +752,9 @@ __device__ void init_params(const DevProblem &P, double freq, int m, double *a) 
 // workgroup; on 0 the solution is in x (LDS or global, n entries).
 // LDS: 2 * n * (n+1) + 16 doubles from buf.
 __device__ inline void pivot_max(double &bv, int &be, double ov, int oe) {
-  if (ov > bv || (ov == bv && oe > be)) {
-    bv = ov;
-    be = oe;
-  }
+  const bool take = (ov > bv) | ((ov == bv) & (oe > be));  // no short-circuit: selects, not branches
+  bv = take ? ov : bv;
+  be = take ? oe : be;
 }
 
 __device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
@@ -860,13 +859,32 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-// Pivot candidate (value, row-major index) reduced over the wave: a
-// butterfly in which lanes l and l^o combine the same two candidates, so
-// every lane ends with the same winner (larger value; equal values: larger
-// index, the reference's ">=" scan keeps the last).
+// One DPP exchange of a pivot candidate: the partner lane's (value, index).
+template <int CTRL>
+__device__ __forceinline__ void dpp_pivot(double &bv, int &be) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(bv), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(bv), CTRL, 0xF, 0xF, false);
+  const int oe = __builtin_amdgcn_update_dpp(0, be, CTRL, 0xF, 0xF, false);
+  pivot_max(bv, be, __hiloint2double(hi, lo), oe);
+}
+
+// Pivot candidate (value, row-major index) reduced over the wave: larger
+// value; equal values: larger index (the reference's ">=" scan keeps the
+// last).  Inside each row of 16 lanes four DPP exchanges pair every lane
+// with one partner (xor 1, xor 2, 7 - i, 15 - i), both lanes of a pair
+// combine the same two candidates, so the row ends uniform; the four rows
+// then meet through scalar registers.  The result is wave-uniform.
 __device__ __forceinline__ void wave_pivot_max(double &bv, int &be) {
-  for (int o = 32; o > 0; o >>= 1) pivot_max(bv, be, __shfl_xor(bv, o, 64), __shfl_xor(be, o, 64));
-  be = __builtin_amdgcn_readfirstlane(be);
+  dpp_pivot<0xB1>(bv, be);   // quad_perm [1,0,3,2]
+  dpp_pivot<0x4E>(bv, be);   // quad_perm [2,3,0,1]
+  dpp_pivot<0x141>(bv, be);  // row_half_mirror
+  dpp_pivot<0x140>(bv, be);  // row_mirror
+  double v = lane_value(bv, 0);
+  int e = __builtin_amdgcn_readlane(be, 0);
+#pragma unroll
+  for (int r = 16; r < 64; r += 16) pivot_max(v, e, lane_value(bv, r), __builtin_amdgcn_readlane(be, r));
+  bv = v;
+  be = e;
 }
 
 // gauss_errc (period_search/gauss_errc.c:9-73) by one wave with the
@@ -891,14 +909,11 @@ __device__ __forceinline__ int gauss_solve_wave(double (&a)[R], int n, double *x
   double bv = -1.0;
   int be = -1;
 #pragma unroll
-  for (int p = 0; p < R; p++) {
-    if (p < n && colc) {  // physical = logical order here: indices increase with p
-      const double v = fabs(a[p]);
-      if (v >= bv) {
-        bv = v;
-        be = p * n + lane;
-      }
-    }
+  for (int p = 0; p < R; p++) {  // physical = logical order here: indices increase with p
+    const double v = fabs(a[p]);
+    const bool take = colc & (p < n) & (v >= bv);
+    bv = take ? v : bv;
+    be = take ? p * n + lane : be;
   }
   wave_pivot_max(bv, be);
   int prev_irow = -1, prev_icol = -1;
@@ -932,22 +947,20 @@ __device__ __forceinline__ int gauss_solve_wave(double (&a)[R], int n, double *x
     const bool candc = colc && !((done >> lane) & 1ull);
     bv = -1.0;
     be = -1;
+    // rows p >= n (register padding) are updated too, harmlessly, but are
+    // never candidates; no branches in the row loop
 #pragma unroll
     for (int p = 0; p < R; p++) {
-      if (p < n) {
-        const double dum = lane_value(a[p], icol);
-        const double v = (p == pp) ? pr : a[p] - pr * dum;
-        a[p] = v;
-        const int j = __builtin_amdgcn_readlane(inv, p);
-        if (candc && !((done >> j) & 1ull)) {
-          const double av = fabs(v);
-          const int idx = j * n + lane;
-          if (av > bv || (av == bv && idx > be)) {
-            bv = av;
-            be = idx;
-          }
-        }
-      }
+      const double dum = lane_value(a[p], icol);
+      const double v = (p == pp) ? pr : a[p] - pr * dum;
+      a[p] = v;
+      const int j = __builtin_amdgcn_readlane(inv, p);
+      const bool cand = candc & (p < n) & !((done >> j) & 1ull);
+      const double av = fabs(v);
+      const int idx = j * n + lane;
+      const bool take = cand & ((av > bv) | ((av == bv) & (idx > be)));
+      bv = take ? av : bv;
+      be = take ? idx : be;
     }
     wave_pivot_max(bv, be);
   }
@@ -988,10 +1001,11 @@ __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, 
 // WAVE (mfit <= 63): one wave per slot, the solve in registers
 // (gauss_solve_wave); otherwise 256 threads and the LDS solver.
 constexpr int kWaveSolveMax = 63;
-template <bool MULTI, bool WAVE>
-__global__ void __launch_bounds__(WAVE ? 64 : kThreads, WAVE ? 1 : 4) step_kernel(DevProblem P0,
-                                                                            const DevProblem *__restrict__ probs,
-                                                                            DevWork W) {
+template <bool MULTI, int RW>  // RW: register rows of the one-wave solver (>= mfit), 0 = LDS solver
+__global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : 4) step_kernel(DevProblem P0,
+                                                                        const DevProblem *__restrict__ probs,
+                                                                        DevWork W) {
+  constexpr bool WAVE = RW > 0;
   constexpr int kStepThreads = WAVE ? 64 : kThreads;
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
@@ -1104,9 +1118,9 @@ __global__ void __launch_bounds__(WAVE ? 64 : kThreads, WAVE ? 1 : 4) step_kerne
     // lanes <= mfit read alpha column min(lane, mfit-1); lane mfit takes beta
     const double *colp = (lane == mfit) ? bacc : aacc + min(lane, mfit - 1);
     const int rstride = (lane == mfit) ? 1 : mfit;
-    double am[kWaveSolveMax + 1];
+    double am[WAVE ? RW : 1];
 #pragma unroll
-    for (int p = 0; p <= kWaveSolveMax; p++) {
+    for (int p = 0; p < RW; p++) {
       const int j = min(p, mfit - 1);
       const double av = colp[j * rstride];
       am[p] = av * ((p == lane) ? (1 + lam) : 1.0);  // times 1.0 is exact; beta's lane never matches
@@ -1938,6 +1952,21 @@ size_t neq_lds(int mfit, bool mix) {
          sizeof(double);
 }
 
+using StepFn = void (*)(DevProblem, const DevProblem *, DevWork);
+// Solver kernel for mfit free parameters: the one-wave register solver with
+// the smallest row count that holds them, or (mfit = 0 here) the LDS solver.
+template <bool MULTI>
+StepFn pick_step(int mfit) {
+  if (mfit <= 0) return step_kernel<MULTI, 0>;
+  if (mfit <= 16) return step_kernel<MULTI, 16>;
+  if (mfit <= 24) return step_kernel<MULTI, 24>;
+  if (mfit <= 32) return step_kernel<MULTI, 32>;
+  if (mfit <= 40) return step_kernel<MULTI, 40>;
+  if (mfit <= 48) return step_kernel<MULTI, 48>;
+  if (mfit <= 56) return step_kernel<MULTI, 56>;
+  return step_kernel<MULTI, 64>;
+}
+
 using NeqFn = void (*)(DevProblem, const DevProblem *, DevWork);
 template <bool MIX, bool MULTI>
 NeqFn pick_neq(int mfit) {
@@ -2605,8 +2634,7 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;
   }
-  e->step = wave_solve ? (multi ? step_kernel<true, true> : step_kernel<false, true>)
-                       : (multi ? step_kernel<true, false> : step_kernel<false, false>);
+  e->step = multi ? pick_step<true>(wave_solve ? mfit : 0) : pick_step<false>(wave_solve ? mfit : 0);
   e->step_threads = wave_solve ? 64 : kThreads;
   e->curv = multi ? curv_kernel<true> : curv_kernel<false>;
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(e->step), hipFuncAttributeMaxDynamicSharedMemorySize,
