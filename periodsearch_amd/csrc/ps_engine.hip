@@ -659,24 +659,51 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
         const JnT s2 = (JnT)aux[row * 3], wg = (JnT)aux[row * 3 + 1];
 #pragma unroll
         for (int i = 0; i < NT; i++) {
-          if (wave + 4 * i < ntile) {
-            const JnT a = Jn[row * MS + 16 * tj[i] + (lane & 15)] * s2 * wg;
-            const JnT b = Jn[row * MS + 16 * tk[i] + (lane & 15)];
-            if constexpr (MIX)
-              acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
-            else
-              acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
-          }
+          // a wave's tiles past the triangle multiply zeros (no branch, the
+          // accumulators stay put); they are never stored
+          const bool act = wave + 4 * i < ntile;
+          const int cj = act ? tj[i] : 0, ck = act ? tk[i] : 0;
+          const JnT av = Jn[row * MS + 16 * cj + (lane & 15)] * s2 * wg;
+          const JnT a = act ? av : (JnT)0;
+          const JnT b = Jn[row * MS + 16 * ck + (lane & 15)];
+          if constexpr (MIX)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
+          else
+            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
         }
       }
+      // beta and chi2 in point order; four points' LDS reads are issued
+      // before their (sequential) sums
       if (tid < mfit) {
-        for (int pp = 0; pp < cn; pp++) {
+        int pp = 0;
+        for (; pp + 4 <= cn; pp += 4) {
+          double wt[4], dy[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            wt[u] = (double)Jn[(pp + u) * MS + tid] * aux[(pp + u) * 3] * aux[(pp + u) * 3 + 1];
+            dy[u] = aux[(pp + u) * 3 + 2];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) bacc = bacc + dy[u] * wt[u];
+        }
+        for (; pp < cn; pp++) {
           const double wt = (double)Jn[pp * MS + tid] * aux[pp * 3] * aux[pp * 3 + 1];
           bacc = bacc + aux[pp * 3 + 2] * wt;
         }
       }
       if (tid == kThreads - 1) {
-        for (int pp = 0; pp < cn; pp++) {
+        int pp = 0;
+        for (; pp + 4 <= cn; pp += 4) {
+          double t[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const double dy = aux[(pp + u) * 3 + 2];
+            t[u] = dy * dy * aux[(pp + u) * 3] * aux[(pp + u) * 3 + 1];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) chi = chi + t[u];
+        }
+        for (; pp < cn; pp++) {
           const double dy = aux[pp * 3 + 2];
           chi = chi + dy * dy * aux[pp * 3] * aux[pp * 3 + 1];
         }
