@@ -21,11 +21,16 @@ CASES = [
 ]
 
 
-def _inputs(tmp_path):
+# lmax 4 and 6 only: at most 54 free parameters, so the several-asteroid
+# kernels run the one-wave register solver (CASES, with lmax 8, the LDS solver)
+CASES_WAVE = [CASES[0], CASES[1], dict(CASES[0], seed=14, period_h=9.2, stop=0.001)]
+
+
+def _inputs(tmp_path, cases=CASES):
     from periodsearch_amd import Input, synth
 
     inps = []
-    for k, c in enumerate(CASES):
+    for k, c in enumerate(cases):
         c = dict(c)
         stop = c.pop("stop")
         ast = synth.make_asteroid(**c)
@@ -35,12 +40,13 @@ def _inputs(tmp_path):
     return inps
 
 
-@pytest.mark.parametrize("prec_name", ["fp64", "bf16x3"])
-def test_multi_problem_batch_equals_single_runs(tmp_path, prec_name):
+@pytest.mark.parametrize("prec_name,cases", [("fp64", "lds"), ("bf16x3", "lds"), ("fp64", "wave"),
+                                              ("bf16x3", "wave")])
+def test_multi_problem_batch_equals_single_runs(tmp_path, prec_name, cases):
     from periodsearch_amd import PS_PREC_BF16X3, PS_PREC_FP64, Engine, PsError
 
     prec = PS_PREC_FP64 if prec_name == "fp64" else PS_PREC_BF16X3
-    inps = _inputs(str(tmp_path))
+    inps = _inputs(str(tmp_path), CASES if cases == "lds" else CASES_WAVE)
     items = []  # (problem, freq, pole)
     for k, inp in enumerate(inps):
         for n in (3, 9):

@@ -249,3 +249,38 @@ def test_graph_replay_matches_plain_launches(small_incl_input, monkeypatch, grou
     for a, b in zip(out["1"], out["0"]):
         assert np.array_equal(a, b)
     assert np.array_equal(out["1"][0], out["1"][1])
+
+
+# (lmax, mmax) -> free parameters ncoef + 5: 14, 24, 30, 35, 48, 57, 63 (one-wave
+# register solver with 16, 24, 32, 40, 48, 64, 64 rows) and 67 (LDS solver)
+SOLVER_SHAPES = [(2, 2), (4, 2), (4, 4), (5, 3), (6, 4), (7, 4), (7, 5), (7, 6)]
+
+
+@pytest.mark.parametrize("lmax,mmax", SOLVER_SHAPES)
+def test_fits_match_oracle_across_solver_sizes(tmp_path, lmax, mmax):
+    """Whole fits at free-parameter counts on both sides of every register-row
+    bucket of the one-wave solver and past it (the LDS solver): ten poles at
+    one trial period against the oracle, which runs the reference's
+    gauss_errc."""
+    import oracle
+    from periodsearch_amd import Engine, Input, synth
+
+    ast = synth.make_asteroid(40 + lmax * 10 + mmax, lmax=lmax, mmax=mmax, nrows=6, n_dense=3, dense_pts=50,
+                              n_sparse=120, noise=0.02, span_years=6, period_h=6.1, incl_deg=8.0)
+    path = str(tmp_path / "period_search_in")
+    synth.write_input(path, ast, 6.0, 6.2)
+    inp = Input(path)
+    eng = Engine(0)
+    eng.set_problem(inp.problem, keep=inp)
+    nc, ma, mfit = eng.dims()
+    assert mfit == nc + 5
+    f = inp.freq_start - 3 * inp.freq_step
+    out = eng.fit_batch([f] * 10, list(range(10)), inp.conw_r0)
+    eng.close()
+    orc = oracle.Oracle(path)
+    orc.prepare(inp.conw_r0)
+    ref = [orc.fit(f, m + 1) for m in range(10)]
+    rel = np.array([_rel(out["chisq"][k], r.chisq) for k, r in enumerate(ref)])
+    print(f"lmax {lmax} mmax {mmax} mfit {mfit}: chi2 rel median {np.median(rel):.1e} max {rel.max():.1e}")
+    assert all(out["niter"][k] == r.niter for k, r in enumerate(ref))
+    assert np.median(rel) < 1e-9 and rel.max() < 1e-5
