@@ -29,6 +29,7 @@
 // (period_search_BOINC.cpp:913-924).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -2005,6 +2006,31 @@ NeqFn pick_neq(int mfit) {
   return nullptr;
 }
 
+using CurvFn = void (*)(DevProblem, const DevProblem *, DevWork);
+
+// The kernels of one LM iteration, chosen for a set of problems: their
+// largest coefficient count and free-parameter count pick the instances,
+// their largest facet/tile counts size the grids.
+struct KernelSet {
+  ModelFn model = nullptr;
+  NeqFn neq = nullptr;
+  StepFn step = nullptr;
+  CurvFn curv = nullptr;
+  size_t lds_step = 0, lds_model = 0, lds_neq = 0;
+  int step_threads = kThreads;
+  bool use_mfma = true;
+  DevProblem G{};  // maxima over the set's problems
+};
+
+// Size class of a problem: the kernel instances it needs (model column
+// blocks, normal-equation tiles per wave, solver rows).
+inline long long size_class(int ncoef, int mfit) {
+  const int nb = (ncoef + 15) / 16;
+  const int fb = (mfit + 15) / 16, nt = (fb * (fb + 1) / 2 + 3) / 4;
+  const int rw = mfit > kWaveSolveMax ? 0 : (mfit <= 16 ? 16 : (mfit + 7) / 8 * 8);
+  return ((long long)nb * 64 + nt) * 128 + rw;
+}
+
 }  // namespace
 
 struct ps_engine {
@@ -2032,6 +2058,12 @@ struct ps_engine {
   size_t feat_cap = 0;
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
   int step_threads = 256;  // one wave when the solve fits in registers
+  // several problems of more than one size class (at most one class per
+  // stream): one kernel set per class, each class with its own slots, item
+  // range and stream; empty otherwise
+  std::vector<KernelSet> cls;
+  std::vector<int> prob_cls;
+  std::vector<int> cls_start;  // item range of each class in the current batch (items sorted by class)
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
   void (*step)(DevProblem, const DevProblem *, DevWork) = nullptr;
@@ -2081,6 +2113,74 @@ void free_problem(ps_engine *e) {
   e->have_problem = false;
   e->d_probs = nullptr;
   e->n_problems = 0;
+  e->cls.clear();
+  e->prob_cls.clear();
+}
+
+// ncoef and mfit of a problem (as upload_problem counts them), for its size class
+void problem_dims(const ps_problem &p, int &ncoef, int &mfit) {
+  ncoef = 0;
+  for (int m = 0; m <= p.mmax; m++) ncoef += (p.lmax - m + 1) * (m ? 2 : 1);
+  mfit = ncoef - (p.onlyrel ? 1 : 0) + 2 + (p.ia_prd ? 1 : 0) + (p.ia_par[0] ? 1 : 0) + (p.ia_par[1] ? 1 : 0) +
+         (p.ia_par[2] ? 1 : 0) + (p.ia_cl ? 1 : 0);
+}
+
+// Kernel instances, LDS sizes and grids for problems with maxima G.
+int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool allow_valu, KernelSet &ks) {
+  const int F = G.nfac, ncoef = G.ncoef, mfit = G.mfit;
+  const bool mixed = e->precision != PS_PREC_FP64;
+  const int mode = e->precision == PS_PREC_BF16 ? 1 : (e->precision == PS_PREC_BF16X3 ? 2 : 0);
+  ks.G = G;
+  ks.use_mfma = mixed || !allow_valu;
+  if (!mixed)
+    ks.model = !ks.use_mfma ? pick_model<false>(G.ncp) : (multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef));
+  else if (multi)
+    ks.model = mode == 1 ? pick_mixed<1, true>(ncoef) : (mode == 2 ? pick_mixed<2, true>(ncoef) : pick_mixed<0, true>(ncoef));
+  else
+    ks.model = mode == 1 ? pick_mixed<1, false>(ncoef) : (mode == 2 ? pick_mixed<2, false>(ncoef) : pick_mixed<0, false>(ncoef));
+  if (!ks.model) {
+    e->err = "no model kernel for this coefficient count";
+    return PS_ERR_LIMIT;
+  }
+  const bool wave_solve = mfit <= kWaveSolveMax;
+  ks.lds_step = step_lds(wave_solve ? 0 : mfit, F);
+  ks.lds_model = mixed ? mixed_layout(F, nb, mode).total : (ks.use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
+  ks.lds_neq = neq_lds(mfit, mixed);
+  const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
+  if (ks.lds_step > lds_max || ks.lds_neq > lds_max || ks.lds_model > lds_max) {
+    e->err = "problem needs more LDS than a CU has";
+    return PS_ERR_LIMIT;
+  }
+  ks.step = multi ? pick_step<true>(wave_solve ? mfit : 0) : pick_step<false>(wave_solve ? mfit : 0);
+  ks.step_threads = wave_solve ? 64 : kThreads;
+  ks.curv = multi ? curv_kernel<true> : curv_kernel<false>;
+  ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit) : pick_neq<true, false>(mfit))
+                 : (multi ? pick_neq<false, true>(mfit) : pick_neq<false, false>(mfit));
+  if (!ks.neq) {
+    e->err = "too many free parameters";
+    return PS_ERR_LIMIT;
+  }
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(ks.step), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ks.lds_step));
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(ks.neq), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ks.lds_neq));
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(ks.model), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ks.lds_model));
+  return PS_OK;
+}
+
+void grow(DevProblem &G, const DevProblem &D) {
+  G.nfac = std::max(G.nfac, D.nfac);
+  G.ncoef = std::max(G.ncoef, D.ncoef);
+  G.ncp = std::max(G.ncp, D.ncp);
+  G.ma = std::max(G.ma, D.ma);
+  G.mfit = std::max(G.mfit, D.mfit);
+  G.npad = std::max(G.npad, D.npad);
+  G.n_lc = std::max(G.n_lc, D.n_lc);
+  G.ntiles = std::max(G.ntiles, D.ntiles);
+  G.ntiles64 = std::max(G.ntiles64, D.ntiles64);
+  G.n_iter_max = std::max(G.n_iter_max, D.n_iter_max);
+  G.n_iter_min = std::max(G.n_iter_min, D.n_iter_min);
 }
 
 int ensure_cg(ps_engine *e, size_t n) {
@@ -2142,40 +2242,84 @@ int run_batch(ps_engine *e, DevWork W) {
   // as few refill rounds as max_slots allows, with the items spread evenly
   // over them (fits of equal length then finish together, no thin tail)
   const int nrounds = (W.n_items + e->max_slots - 1) / e->max_slots;
-  const int nslots = (W.n_items + nrounds - 1) / nrounds;
-  int rc;
-  if ((rc = ensure_slots(e, nslots))) return rc;
-  W.nslots = nslots;
-  W.queue = e->counters;
-  W.done = e->counters + 1;
-  W.st = e->st;
-  W.mem = e->mem;
-  W.slot_doubles = e->slot_doubles;
-  const DevProblem &P = e->G;  // grid sizes and iteration bound: maxima over the problems
   const DevProblem &P0 = e->P;  // the problem when there is one
   const DevProblem *probs = e->d_probs;
-  hipLaunchKernelGGL(reset_batch, dim3((nslots + 255) / 256), dim3(256), 0, e->stream, e->st, nslots, W.queue,
-                     W.done);
+  // Lanes: each drives its own slots with its own kernel set on its own
+  // stream.  Size classes (several problems of different sizes): one lane
+  // per class over its contiguous item range, with its own queue and done
+  // counters.  Otherwise the slots are split into up to `groups` lanes that
+  // share the item queue and the done counters (atomics), so the
+  // latency-bound per-slot kernels of one lane overlap the model kernels of
+  // the others.
+  struct Lane {
+    const KernelSet *ks;
+    DevWork W;
+  };
+  KernelSet eks;
+  eks.model = e->model;
+  eks.neq = e->neq;
+  eks.step = e->step;
+  eks.curv = e->curv;
+  eks.lds_step = e->lds_step;
+  eks.lds_model = e->lds_model;
+  eks.lds_neq = e->lds_neq;
+  eks.step_threads = e->step_threads;
+  eks.use_mfma = e->use_mfma;
+  eks.G = e->G;
+  Lane lane[kMaxGroups];
+  int nl = 0, nslots = 0;
+  const bool by_class = !e->cls.empty() && W.item_prob && e->cls_start.size() == e->cls.size() + 1;
+  if (by_class) {
+    for (size_t c = 0; c < e->cls.size(); c++) {
+      const int n_c = e->cls_start[c + 1] - e->cls_start[c];
+      if (n_c <= 0) continue;
+      Lane &L = lane[nl];
+      L.ks = &e->cls[c];
+      L.W = W;
+      L.W.n_items = n_c;
+      const int start = e->cls_start[c];
+      L.W.freq = W.freq + start;
+      L.W.pole = W.pole + start;
+      L.W.item_prob = W.item_prob + start;
+      L.W.out = W.out + start;
+      L.W.nslots = (n_c + nrounds - 1) / nrounds;
+      L.W.queue = e->counters + 3 * nl;  // per lane: queue, fits done, evaluations
+      L.W.done = e->counters + 3 * nl + 1;
+      nslots += L.W.nslots;
+      nl++;
+    }
+  } else {
+    nslots = (W.n_items + nrounds - 1) / nrounds;
+    int ng = e->groups;
+    while (ng > 1 && nslots < ng * 2 * kWaves) ng--;
+    for (int g = 0; g < ng; g++) {
+      lane[g].ks = &eks;
+      lane[g].W = W;
+      lane[g].W.nslots = nslots / ng + (g < nslots % ng ? 1 : 0);
+      lane[g].W.queue = e->counters;
+      lane[g].W.done = e->counters + 1;
+    }
+    nl = ng;
+  }
+  int rc;
+  if ((rc = ensure_slots(e, nslots))) return rc;
+  for (int l = 0, base = 0; l < nl; l++) {
+    lane[l].W.st = e->st + base;
+    lane[l].W.mem = e->mem + (size_t)base * e->slot_doubles;
+    lane[l].W.slot_doubles = e->slot_doubles;
+    base += lane[l].W.nslots;
+  }
+  // clear every lane's slots and counters (shared counters: the same values again)
+  for (int l = 0; l < nl; l++)
+    hipLaunchKernelGGL(reset_batch, dim3((lane[l].W.nslots + 255) / 256), dim3(256), 0, e->stream, lane[l].W.st,
+                       lane[l].W.nslots, lane[l].W.queue, lane[l].W.done);
   HIP_TRY(hipGetLastError());
-  // The slots are split into groups, each driven on its own stream: the
-  // kernels of different groups are independent (they share only the item
-  // queue and the done counters, both atomic), so the latency-bound
-  // per-slot kernels of one group overlap the model kernels of the others.
-  int ng = e->groups;
-  while (ng > 1 && nslots < ng * 2 * kWaves) ng--;
-  DevWork Wg[kMaxGroups];
   hipStream_t *sg = e->streams;
   HIP_TRY(hipEventRecord(e->evg[0], sg[0]));
-  for (int g = 0, base = 0; g < ng; g++) {
-    Wg[g] = W;
-    Wg[g].nslots = nslots / ng + (g < nslots % ng ? 1 : 0);
-    Wg[g].st = e->st + base;
-    Wg[g].mem = e->mem + (size_t)base * e->slot_doubles;
-    base += Wg[g].nslots;
-    if (g > 0) HIP_TRY(hipStreamWaitEvent(sg[g], e->evg[0], 0));
-  }
+  for (int l = 1; l < nl; l++) HIP_TRY(hipStreamWaitEvent(sg[l], e->evg[0], 0));
+  const DevProblem &P = e->G;  // iteration bound: maxima over the problems
   const int per_fit = (P.n_iter_max > P.n_iter_min ? P.n_iter_max : P.n_iter_min) + 3;
-  const long long rounds = (W.n_items + nslots - 1) / nslots + 1;
+  const long long rounds = nrounds + 1;
   const long long guard = rounds * per_fit * 2 + 64;
   const int poll = 8;
   if (e->profile && e->pev.size() < (size_t)(2 * kMaxGroups * poll)) {
@@ -2183,29 +2327,30 @@ int run_batch(ps_engine *e, DevWork W) {
     e->pev.assign(2 * kMaxGroups * poll, nullptr);
     for (auto &ev : e->pev) HIP_TRY(hipEventCreate(&ev));
   }
-  // One poll = `poll` LM iterations of every group, then the groups join
+  // One poll = `poll` LM iterations of every lane, then the lanes join
   // stream 0.  Without profiling events the poll is captured once per batch
   // into a HIP graph (its kernel arguments do not change between polls) and
   // replayed; profiling records events between the launches instead.
   auto enqueue_poll = [&](bool prof) -> int {
     for (int k = 0; k < poll; k++) {
-      for (int g = 0; g < ng; g++) {
-        const DevWork &Wk = Wg[g];
+      for (int l = 0; l < nl; l++) {
+        const DevWork &Wk = lane[l].W;
+        const KernelSet &K = *lane[l].ks;
         const int ns = Wk.nslots;
-        hipLaunchKernelGGL(e->step, dim3(ns), dim3(e->step_threads), e->lds_step, sg[g], P0, probs, Wk);
-        hipLaunchKernelGGL(e->curv, dim3(ns * ((P.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[g], P0,
+        hipLaunchKernelGGL(K.step, dim3(ns), dim3(K.step_threads), K.lds_step, sg[l], P0, probs, Wk);
+        hipLaunchKernelGGL(K.curv, dim3(ns * ((K.G.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[l], P0,
                            probs, Wk);
-        if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g)], sg[g]));
-        hipLaunchKernelGGL(e->model, dim3(ns * (e->use_mfma ? P.ntiles64 : P.ntiles)), dim3(kThreads), e->lds_model,
-                           sg[g], P0, probs, Wk);
-        if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + g) + 1], sg[g]));
-        hipLaunchKernelGGL(e->neq, dim3(ns), dim3(kThreads), e->lds_neq, sg[g], P0, probs, Wk);
+        if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + l)], sg[l]));
+        hipLaunchKernelGGL(K.model, dim3(ns * (K.use_mfma ? K.G.ntiles64 : K.G.ntiles)), dim3(kThreads),
+                           K.lds_model, sg[l], P0, probs, Wk);
+        if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + l) + 1], sg[l]));
+        hipLaunchKernelGGL(K.neq, dim3(ns), dim3(kThreads), K.lds_neq, sg[l], P0, probs, Wk);
       }
     }
     HIP_TRY(hipGetLastError());
-    for (int g = 1; g < ng; g++) {
-      HIP_TRY(hipEventRecord(e->evg[g], sg[g]));
-      HIP_TRY(hipStreamWaitEvent(sg[0], e->evg[g], 0));
+    for (int l = 1; l < nl; l++) {
+      HIP_TRY(hipEventRecord(e->evg[l], sg[l]));
+      HIP_TRY(hipStreamWaitEvent(sg[0], e->evg[l], 0));
     }
     return PS_OK;
   };
@@ -2213,8 +2358,8 @@ int run_batch(ps_engine *e, DevWork W) {
   if (e->use_graph && !e->profile) {
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(sg[0], hipStreamCaptureModeThreadLocal));
-    HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other groups' streams into the capture
-    for (int g = 1; g < ng; g++) HIP_TRY(hipStreamWaitEvent(sg[g], e->evf, 0));
+    HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other lanes' streams into the capture
+    for (int l = 1; l < nl; l++) HIP_TRY(hipStreamWaitEvent(sg[l], e->evf, 0));
     int rc_cap = enqueue_poll(false);
     hipError_t err_end = hipStreamEndCapture(sg[0], &graph);
     if (rc_cap) return rc_cap;
@@ -2229,6 +2374,7 @@ int run_batch(ps_engine *e, DevWork W) {
       if (g) (void)hipGraphExecDestroy(g);
     }
   } guard_graph{gexec};
+  const int ncnt = by_class ? 3 * nl : 3;
   HIP_TRY(hipEventRecord(e->ev0, e->stream));
   long long it = 0;
   while (true) {
@@ -2239,18 +2385,25 @@ int run_batch(ps_engine *e, DevWork W) {
       if (rc_p) return rc_p;
     }
     it += poll;
-    HIP_TRY(hipMemcpyAsync(e->host_done, W.done, 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->host_done, e->counters, ncnt * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->profile) {
       for (int k = 0; k < poll; k++)
-        for (int g = 0; g < ng; g++) {
+        for (int l = 0; l < nl; l++) {
           float ms = 0;
-          HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + g)], e->pev[2 * (kMaxGroups * k + g) + 1]));
+          HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + l)], e->pev[2 * (kMaxGroups * k + l) + 1]));
           e->model_ms += ms;
         }
-      e->model_launches += poll * ng;
+      e->model_launches += poll * nl;
     }
-    if (*e->host_done >= W.n_items) break;
+    // host_done: queue, fits done, evaluations (per lane with size classes)
+    bool all = true;
+    if (by_class) {
+      for (int l = 0; l < nl; l++) all = all && e->host_done[3 * l + 1] >= lane[l].W.n_items;
+    } else {
+      all = e->host_done[1] >= W.n_items;
+    }
+    if (all) break;
     if (it > guard) {
       e->err = "fit batch did not finish within its iteration bound";
       return PS_ERR_STATE;
@@ -2263,7 +2416,7 @@ int run_batch(ps_engine *e, DevWork W) {
   e->last_ms = ms;
   e->last_items = W.n_items;
   e->last_iters = it;
-  e->evals += e->host_done[1];
+  for (int l = 0; l < (by_class ? nl : 1); l++) e->evals += e->host_done[3 * l + 2];
   return PS_OK;
 }
 
@@ -2508,8 +2661,9 @@ int ps_create(int device, ps_engine **out) {
   e->stream = e->streams[0];
   if (!ok || hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
       hipEventCreateWithFlags(&e->evf, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc(&e->counters, 4 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipMalloc(&e->counters, 3 * kMaxGroups * sizeof(int)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&e->host_done), 3 * kMaxGroups * sizeof(int),
+                    hipHostMallocDefault) != hipSuccess) {
     delete e;
     return PS_ERR_HIP;
   }
@@ -2581,24 +2735,37 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
   HIP_TRY(hipSetDevice(e->device));
   HIP_TRY(hipStreamSynchronize(e->stream));
   free_problem(e);
-  // the largest coefficient count decides the kernel instances and tables
+  // size classes: with 2..kMaxGroups of them each class gets its own kernel
+  // instances (its problems' tables built for them); otherwise the largest
+  // coefficient count decides the instances and the tables of every problem
+  std::vector<long long> key(n);
+  std::vector<int> pnc(n), pmf(n);
   int nc_max = 0;
   for (int i = 0; i < n; i++) {
     if (p[i].lmax < 0 || p[i].mmax < 0 || p[i].mmax > p[i].lmax) {
       e->err = "invalid problem";
       return PS_ERR_ARG;
     }
-    int nc = 0;
-    for (int m = 0; m <= p[i].mmax; m++) nc += (p[i].lmax - m + 1) * (m ? 2 : 1);
-    nc_max = nc > nc_max ? nc : nc_max;
+    problem_dims(p[i], pnc[i], pmf[i]);
+    key[i] = size_class(pnc[i], pmf[i]);
+    nc_max = std::max(nc_max, pnc[i]);
   }
-  const int nb = (nc_max + 15) / 16;
+  std::vector<long long> keys(key);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const bool by_class = n > 1 && keys.size() > 1 && keys.size() <= (size_t)kMaxGroups && getenv("PSAMD_NOCLASS") == nullptr;
+  std::vector<int> cls(n, 0);
+  for (int i = 0; i < n; i++) cls[i] = (int)(std::lower_bound(keys.begin(), keys.end(), key[i]) - keys.begin());
+  const int nb_all = (nc_max + 15) / 16;
   std::vector<DevProblem> hp(n);
-  DevProblem G{};  // maxima over the batch (grids, slot size, LDS)
+  DevProblem G{};  // maxima over the batch (grids, slot size, iteration bound)
+  std::vector<DevProblem> Gc(keys.size());
+  std::vector<bool> seen(keys.size(), false);
   int rc;
   for (int i = 0; i < n; i++) {
     float feat[2];
-    if ((rc = upload_problem(e, p + i, nb, hp[i], feat))) {
+    const int nb_tab = by_class ? (pnc[i] + 15) / 16 : nb_all;
+    if ((rc = upload_problem(e, p + i, nb_tab, hp[i], feat))) {
       free_problem(e);
       return rc;
     }
@@ -2607,36 +2774,13 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
       e->feat_aspect = feat[1];
       G = hp[0];
     }
-    const DevProblem &D = hp[i];
-    G.nfac = std::max(G.nfac, D.nfac);
-    G.ncoef = std::max(G.ncoef, D.ncoef);
-    G.ncp = std::max(G.ncp, D.ncp);
-    G.ma = std::max(G.ma, D.ma);
-    G.mfit = std::max(G.mfit, D.mfit);
-    G.npad = std::max(G.npad, D.npad);
-    G.n_lc = std::max(G.n_lc, D.n_lc);
-    G.ntiles = std::max(G.ntiles, D.ntiles);
-    G.ntiles64 = std::max(G.ntiles64, D.ntiles64);
-    G.n_iter_max = std::max(G.n_iter_max, D.n_iter_max);
-    G.n_iter_min = std::max(G.n_iter_min, D.n_iter_min);
-  }
-  const int F = G.nfac, ncoef = G.ncoef, mfit = G.mfit;
-  const bool mixed = e->precision != PS_PREC_FP64;
-  const int mode = e->precision == PS_PREC_BF16 ? 1 : (e->precision == PS_PREC_BF16X3 ? 2 : 0);
-  const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs, one problem)
-  e->use_mfma = mixed || n > 1 || !(kv && !strcmp(kv, "valu"));
-  const bool multi = n > 1;
-  ModelFn mk;
-  if (!mixed)
-    mk = !e->use_mfma ? pick_model<false>(G.ncp) : (multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef));
-  else if (multi)
-    mk = mode == 1 ? pick_mixed<1, true>(ncoef) : (mode == 2 ? pick_mixed<2, true>(ncoef) : pick_mixed<0, true>(ncoef));
-  else
-    mk = mode == 1 ? pick_mixed<1, false>(ncoef) : (mode == 2 ? pick_mixed<2, false>(ncoef) : pick_mixed<0, false>(ncoef));
-  if (!mk) {
-    free_problem(e);
-    e->err = "no model kernel for this coefficient count";
-    return PS_ERR_LIMIT;
+    grow(G, hp[i]);
+    const int c = cls[i];
+    if (!seen[c]) {
+      Gc[c] = hp[i];
+      seen[c] = true;
+    }
+    grow(Gc[c], hp[i]);
   }
   const DevProblem *dp = nullptr;
   if ((rc = upload(e, hp.data(), (size_t)n, &dp)) != PS_OK) {
@@ -2650,33 +2794,31 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
   e->n_poles = p[0].n_poles;
   e->prob_npoles.resize(n);
   for (int i = 0; i < n; i++) e->prob_npoles[i] = p[i].n_poles;
-  e->model = mk;
-  const bool wave_solve = mfit <= kWaveSolveMax;
-  e->lds_step = step_lds(wave_solve ? 0 : mfit, F);
-  e->lds_model = mixed ? mixed_layout(F, nb, mode).total : (e->use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
-  e->lds_neq = neq_lds(mfit, mixed);
-  const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
-  if (e->lds_step > lds_max || e->lds_neq > lds_max || e->lds_model > lds_max) {
+  const char *kv = getenv("PSAMD_MODEL");  // "valu" selects the VALU FP64 kernel (comparison runs, one problem)
+  const bool allow_valu = n == 1 && kv && !strcmp(kv, "valu");
+  KernelSet ks;
+  if (by_class) {
+    e->cls.resize(keys.size());
+    for (size_t c = 0; c < keys.size(); c++)
+      if ((rc = build_kernels(e, Gc[c], (Gc[c].ncoef + 15) / 16, true, false, e->cls[c]))) {
+        free_problem(e);
+        return rc;
+      }
+    e->prob_cls = cls;
+    ks = e->cls.back();  // the engine-wide set is not used for several classes; keep a valid one
+  } else if ((rc = build_kernels(e, G, nb_all, n > 1, allow_valu, ks))) {
     free_problem(e);
-    e->err = "problem needs more LDS than a CU has";
-    return PS_ERR_LIMIT;
+    return rc;
   }
-  e->step = multi ? pick_step<true>(wave_solve ? mfit : 0) : pick_step<false>(wave_solve ? mfit : 0);
-  e->step_threads = wave_solve ? 64 : kThreads;
-  e->curv = multi ? curv_kernel<true> : curv_kernel<false>;
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(e->step), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)e->lds_step));
-  e->neq = mixed ? (multi ? pick_neq<true, true>(mfit) : pick_neq<true, false>(mfit))
-                 : (multi ? pick_neq<false, true>(mfit) : pick_neq<false, false>(mfit));
-  if (!e->neq) {
-    free_problem(e);
-    e->err = "too many free parameters";
-    return PS_ERR_LIMIT;
-  }
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(e->neq),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_neq));
-  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(mk), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)e->lds_model));
+  e->model = ks.model;
+  e->neq = ks.neq;
+  e->step = ks.step;
+  e->curv = ks.curv;
+  e->lds_step = ks.lds_step;
+  e->lds_model = ks.lds_model;
+  e->lds_neq = ks.lds_neq;
+  e->step_threads = ks.step_threads;
+  e->use_mfma = ks.use_mfma;
   e->have_problem = true;
   return PS_OK;
 }
@@ -2835,6 +2977,32 @@ int ps_fit_batch_multi(ps_engine *e, const int *prob, const double *freq, const 
     HIP_TRY(hipMalloc(&e->d_conw, (size_t)e->n_problems * sizeof(double)));
     e->conw_cap = e->n_problems;
   }
+  // with size classes the items run grouped by class (each class its own
+  // lane); the results go back to the caller's order
+  std::vector<int> order;
+  std::vector<double> freq_s;
+  std::vector<int> pole_s, prob_s;
+  e->cls_start.clear();
+  if (!e->cls.empty()) {
+    const int nc = (int)e->cls.size();
+    e->cls_start.assign(nc + 1, 0);
+    for (int i = 0; i < n; i++) e->cls_start[e->prob_cls[prob[i]] + 1]++;
+    for (int c = 0; c < nc; c++) e->cls_start[c + 1] += e->cls_start[c];
+    std::vector<int> fill(e->cls_start.begin(), e->cls_start.end() - 1);
+    order.resize(n);
+    for (int i = 0; i < n; i++) order[fill[e->prob_cls[prob[i]]]++] = i;
+    freq_s.resize(n);
+    pole_s.resize(n);
+    prob_s.resize(n);
+    for (int k = 0; k < n; k++) {
+      freq_s[k] = freq[order[k]];
+      pole_s[k] = pole[order[k]];
+      prob_s[k] = prob[order[k]];
+    }
+    freq = freq_s.data();
+    pole = pole_s.data();
+    prob = prob_s.data();
+  }
   HIP_TRY(hipMemcpyAsync(e->d_freq, freq, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
   HIP_TRY(hipMemcpyAsync(e->d_pole, pole, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
   HIP_TRY(hipMemcpyAsync(e->d_iprob, prob, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
@@ -2848,8 +3016,15 @@ int ps_fit_batch_multi(ps_engine *e, const int *prob, const double *freq, const 
   W.conw_rs = e->d_conw;
   W.out = e->d_out;
   if ((rc = run_batch(e, W))) return rc;
-  HIP_TRY(hipMemcpyAsync(out, e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
-  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (order.empty()) {
+    HIP_TRY(hipMemcpyAsync(out, e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+  } else {
+    std::vector<ps_fit_out> tmp(n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int k = 0; k < n; k++) out[order[k]] = tmp[k];
+  }
   return PS_OK;
 }
 
