@@ -40,6 +40,9 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector = matrix), dense
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix, dense
 BF16_PEAK_TFLOPS = 2500.0  # MI355X bf16 matrix, dense
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, published (counts packed FP32)
+FP32_VECTOR_UNPACKED_TFLOPS = 78.6  # the same without packed FP32, which this build does not emit
+FACET_FLOP_PER_PAIR = 40.0  # facet stage per (point, facet) pair: mu, mu0, scattering law, its derivatives, sums
 WORKLOAD = "wu"
 # HBM bytes per model evaluation of the model kernel on this workload, from the
 # FETCH_SIZE / WRITE_SIZE counter runs (copy-calibrated) summarised in
@@ -267,6 +270,17 @@ def main() -> int:
     achieved = (st["evals"] * flop_eval) / (st["model_ms"] * 1e-3) / 1e12 if st["model_ms"] > 0 else 0.0
     peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS,
             "bf16x3": BF16_PEAK_TFLOPS}[args.prec]
+    # the model kernel's other half: the facet stage on the vector units (FP64
+    # in the plain form, FP32 in the mixed forms), which bounds the mixed forms
+    vec_eval = FACET_FLOP_PER_PAIR * N * F
+    vec_achieved = (st["evals"] * vec_eval) / (st["model_ms"] * 1e-3) / 1e12 if st["model_ms"] > 0 else 0.0
+    vec_peak = FP64_PEAK_TFLOPS if args.prec == "fp64" else FP32_VECTOR_PEAK_TFLOPS
+    vector = {"stage": "facet stage, %d flop per point-facet pair" % FACET_FLOP_PER_PAIR,
+              "dtype": "f64" if args.prec == "fp64" else "f32", "achieved": round(vec_achieved, 3),
+              "peak": vec_peak, "unit": "TFLOP/s", "frac": round(vec_achieved / vec_peak, 4)}
+    if args.prec != "fp64":
+        vector["peak_without_packed_fp32"] = FP32_VECTOR_UNPACKED_TFLOPS
+        vector["frac_without_packed_fp32"] = round(vec_achieved / FP32_VECTOR_UNPACKED_TFLOPS, 4)
     iters = 50
     flop_fit = (iters + 1) * (2.0 * N * F * nc + (N + 3) * mfit * (mfit + 1))
     out = {
@@ -296,7 +310,7 @@ def main() -> int:
                      "traffic": round(MODEL_HBM_BYTES_PER_EVAL[args.prec] * st["evals"] / max(st["model_launches"], 1)),
                      "kernel": "model_kernel", "model_ms_total": round(st["model_ms"], 3),
                      "model_launches": st["model_launches"], "evals": st["evals"],
-                     "flop_per_eval": flop_eval},
+                     "flop_per_eval": flop_eval, "vector": vector},
         "job_tflops": round(lm_fits * flop_fit / wall / 1e12, 3),
         "lm_fits_per_s": round(lm_fits / wall, 2),
     }
