@@ -52,11 +52,45 @@ WORKLOAD = "wu"
 MODEL_HBM_BYTES_PER_EVAL = {"fp64": 469.4e3, "fp32": 242.2e3, "bf16": 242.2e3, "bf16x3": 242.2e3}
 
 
-def cpu_baseline(path: str, conw_r: float, n_procs: int, periods_each: int = 1) -> dict:
-    """The oracle (single-threaded, like the reference) on n_procs host cores,
-    each process a disjoint slice of the period index (the BOINC split)."""
+def cpu_baseline_reference(path: str, conw_r: float, n_procs: int, periods_each: int = 1) -> dict | None:
+    """The reference's own fit loop (period_search/*.c compiled unchanged into
+    oracle/_ref, single-threaded like the reference) in n_procs processes, each
+    a disjoint slice of the period index (the BOINC split).  The rate is the
+    fits over the span from the first fit's start to the last fit's end (the
+    processes' start-up and input parsing are outside it)."""
     import oracle
 
+    if not os.path.exists(oracle.REF_LIB):
+        return None
+    procs = []
+    for k in range(n_procs):
+        n0 = 1 + k * periods_each
+        procs.append(subprocess.Popen([sys.executable, "-m", "oracle.ref_bench", path, repr(conw_r), str(n0),
+                                       str(n0 + periods_each - 1)], cwd=ROOT, stdout=subprocess.PIPE,
+                                      stderr=subprocess.DEVNULL, text=True))
+    res = []
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            return None
+        res.append(json.loads(out.strip().splitlines()[-1]))
+    fits = sum(r["fits"] for r in res)
+    span = max(r["t1"] for r in res) - min(r["t0"] for r in res)
+    return {"value": fits / span, "unit": "fits/s", "cores": n_procs, "kind": "reference",
+            "sample": f"{fits} fits: periods 1..{n_procs * periods_each} of the same workload, {n_procs} "
+                      f"single-threaded processes running the reference's period_search/*.c (g++ -O3, unchanged) "
+                      f"through oracle/_ref, {span:.1f} s from the first fit's start to the last fit's end"}
+
+
+def cpu_baseline(path: str, conw_r: float, n_procs: int, periods_each: int = 1) -> dict:
+    """The reference build where oracle/_ref exists, else the oracle
+    (single-threaded, like the reference) on n_procs host cores, each process
+    a disjoint slice of the period index (the BOINC split)."""
+    import oracle
+
+    ref = cpu_baseline_reference(path, conw_r, n_procs, periods_each)
+    if ref is not None:
+        return ref
     exe = oracle.ORACLE_MAIN
     if not os.path.exists(exe):
         return None
