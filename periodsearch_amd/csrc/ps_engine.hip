@@ -779,17 +779,31 @@ __device__ void init_params(const DevProblem &P, double freq, int m, double *a) 
 // permutation.  Returns 0, 1 (singular) or 2 (zero pivot), uniform across the
 // workgroup; on 0 the solution is in x (LDS or global, n entries).
 // LDS: 2 * n * (n+1) + 16 doubles from buf.
+// INPLACE (for n whose two copies exceed the LDS; up to 128 free parameters,
+// lmax = mmax = 10): one copy, n * (n+1) + 16 doubles, updated in place.  In a
+// pivot step the only entries that one thread writes and others read are
+// those of rows irow and icol (the pivot row and its swap partner; column
+// icol is pivoted and never written again), so every thread first reads
+// those two rows at its columns into registers, and a barrier separates
+// these reads from the writes.  The arithmetic and its order are the same.
 __device__ inline void pivot_max(double &bv, int &be, double ov, int oe) {
   const bool take = (ov > bv) | ((ov == bv) & (oe > be));  // no short-circuit: selects, not branches
   bv = take ? ov : bv;
   be = take ? oe : be;
 }
 
+constexpr size_t kSolveLdsDoubles = (160 * 1024 - 1024) / sizeof(double);  // dynamic LDS of the step kernel
+constexpr int kSolveCols = (kMaxMfit + 1 + 63) / 64;                      // columns of [A|b] per lane
+__host__ __device__ inline bool solve_in_place(int n) {
+  return 2 * (size_t)n * (n + 1) + 4 * kWaves > kSolveLdsDoubles;
+}
+
+template <bool INPLACE>
 __device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
 #pragma clang fp contract(off)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ld = n + 1, nn = n * ld;
-  double *red = buf + 2 * nn;  // [2][kWaves] x (value, index)
+  double *red = buf + (INPLACE ? 1 : 2) * nn;  // [2][kWaves] x (value, index)
   unsigned long long done[2] = {0ull, 0ull};  // pivoted columns (ipiv != 0) = pivoted rows
   auto is_done = [&](int k) { return (done[k >> 6] >> (k & 63)) & 1ull; };
   // pivot search over the initial matrix
@@ -813,8 +827,8 @@ __device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
   __syncthreads();
   int prev_irow = -1, prev_icol = -1;
   for (int i = 0; i < n; i++) {
-    const double *Ao = buf + (i & 1) * nn;
-    double *An = buf + ((i + 1) & 1) * nn;
+    const double *Ao = INPLACE ? buf : buf + (i & 1) * nn;
+    double *An = INPLACE ? buf : buf + ((i + 1) & 1) * nn;
     const double *rd = red + (i & 1) * 2 * kWaves;
     double bv = rd[0];
     int be = (int)rd[1];
@@ -837,6 +851,49 @@ __device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
     done[icol >> 6] |= 1ull << (icol & 63);
     double nbv = -1.0;
     int nbe = -1;
+    if constexpr (INPLACE) {
+      // rows irow (pivot row) and icol (its swap partner) at this lane's
+      // columns, read before any thread writes them
+      double prow[kSolveCols], crow[kSolveCols];
+#pragma unroll
+      for (int k = 0; k < kSolveCols; k++) {
+        const int c = min(lane + 64 * k, n);
+        prow[k] = Ao[irow * ld + c];
+        crow[k] = Ao[icol * ld + c];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kSolveCols; k++) {
+        const int c = lane + 64 * k;
+        if (c > n || (c < n && is_done(c))) continue;  // pivoted (incl. icol): never read again
+        const double pr = prow[k] * pivinv;            // new a[icol][c] (b for c == n)
+        for (int j0 = wave; j0 < n; j0 += 8 * kWaves) {
+          double av_[8], ac_[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int j = min(j0 + kWaves * u, n - 1);
+            const int sj = (j == irow) ? icol : j;  // row j after the swap (row icol takes pr below)
+            av_[u] = (j == irow) ? crow[k] : Ao[j * ld + c];
+            ac_[u] = Ao[sj * ld + icol];            // column icol: never written
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int j = j0 + kWaves * u;
+            if (j < n) {
+              const double v = (j == icol) ? pr : av_[u] - pr * ac_[u];
+              An[j * ld + c] = v;
+              if (c < n && !is_done(j)) {
+                const double a = fabs(v);
+                if (a >= nbv) {
+                  nbv = a;
+                  nbe = j * n + c;
+                }
+              }
+            }
+          }
+        }
+      }
+    } else
     for (int c = lane; c <= n; c += 64) {
       if (c < n && is_done(c)) continue;  // pivoted (incl. icol): never read again
       const double pr = Ao[irow * ld + c] * pivinv;  // new a[icol][c] (b for c == n)
@@ -875,7 +932,7 @@ __device__ __forceinline__ int gauss_solve(double *buf, int n, double *x) {
     }
     __syncthreads();
   }
-  const double *Af = buf + (n & 1) * nn;
+  const double *Af = INPLACE ? buf : buf + (n & 1) * nn;
   for (int j = tid; j < n; j += kThreads) x[j] = Af[j * ld + n];
   return 0;
 }
@@ -1182,7 +1239,7 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : 4) step_kernel(De
     }
   }
   __syncthreads();
-  err = gauss_solve(A, mfit, da);
+  err = solve_in_place(mfit) ? gauss_solve<true>(A, mfit, da) : gauss_solve<false>(A, mfit, da);
   __syncthreads();
   }
   double *a = mem + M.a, *atry = mem + M.atry;
@@ -1969,7 +2026,8 @@ size_t mfma_lds(int nfac, int ncoef) {
 }
 
 size_t step_lds(int mfit, int nfac) {  // mfit = 0: the register solver (no LDS for the solve)
-  const size_t solve = mfit ? 2 * (size_t)mfit * (mfit + 1) + 4 * kWaves : 0, fin = 2 * (size_t)nfac;
+  const size_t copies = solve_in_place(mfit) ? 1 : 2;
+  const size_t solve = mfit ? copies * (size_t)mfit * (mfit + 1) + 4 * kWaves : 0, fin = 2 * (size_t)nfac;
   return (solve > fin ? solve : fin) * sizeof(double);
 }
 size_t model_lds(int nfac) {

@@ -252,15 +252,18 @@ def test_graph_replay_matches_plain_launches(small_incl_input, monkeypatch, grou
 
 
 # (lmax, mmax) -> free parameters ncoef + 5: 14, 24, 30, 35, 48, 57, 63 (one-wave
-# register solver with 16, 24, 32, 40, 48, 64, 64 rows) and 67 (LDS solver)
-SOLVER_SHAPES = [(2, 2), (4, 2), (4, 4), (5, 3), (6, 4), (7, 4), (7, 5), (7, 6)]
+# register solver with 16, 24, 32, 40, 48, 64, 64 rows), 67 (LDS solver, two
+# copies of [A|b]) and 105, 126 (LDS solver in place: lmax = mmax = 9 and 10,
+# the reference's largest, period_search/constants.h:8)
+SOLVER_SHAPES = [(2, 2), (4, 2), (4, 4), (5, 3), (6, 4), (7, 4), (7, 5), (7, 6), (9, 9), (10, 10)]
 
 
 @pytest.mark.parametrize("lmax,mmax", SOLVER_SHAPES)
 def test_fits_match_oracle_across_solver_sizes(tmp_path, lmax, mmax):
     """Whole fits at free-parameter counts on both sides of every register-row
-    bucket of the one-wave solver and past it (the LDS solver): ten poles at
-    one trial period against the oracle, which runs the reference's
+    bucket of the one-wave solver and past it (the LDS solver, with two copies
+    of the augmented matrix and, past 100 free parameters, in place): ten
+    poles at one trial period against the oracle, which runs the reference's
     gauss_errc."""
     import oracle
     from periodsearch_amd import Engine, Input, synth
