@@ -287,3 +287,64 @@ def test_fits_match_oracle_across_solver_sizes(tmp_path, lmax, mmax):
     print(f"lmax {lmax} mmax {mmax} mfit {mfit}: chi2 rel median {np.median(rel):.1e} max {rel.max():.1e}")
     assert all(out["niter"][k] == r.niter for k, r in enumerate(ref))
     assert np.median(rel) < 1e-9 and rel.max() < 1e-5
+
+
+@pytest.fixture(scope="module")
+def largest_input(tmp_path_factory):
+    """lmax = mmax = 10 (121 coefficients, 126 free parameters), the
+    reference's largest shape (period_search/constants.h:8), on 288 facets."""
+    from periodsearch_amd import synth
+
+    ast = synth.make_asteroid(151, lmax=10, mmax=10, nrows=6, n_dense=3, dense_pts=50, n_sparse=120, noise=0.02,
+                              span_years=6, period_h=6.1, incl_deg=8.0)
+    p = str(tmp_path_factory.mktemp("largest") / "period_search_in")
+    synth.write_input(p, ast, 6.0, 6.2)
+    return p
+
+
+# Error bounds at lmax 10, relative to the largest entry.  The order-10
+# columns of Dsph (unnormalised P_l^m up to 6.5e8, times cos or sin of m phi)
+# change sign across the facets, so their facet sums cancel and the reduced
+# forms lose more than at lmax 6.  Measured worst of 12 evaluations:
+#   FP32        alpha 1.6e-6  beta 3.4e-6
+#   bf16        alpha 4.2e-2  beta 8.6e-2   (2.8e-3 / 4.1e-2 at lmax 6)
+#   split bf16  alpha 1.1e-4  beta 2.3e-4   (6.3e-6 / 7.9e-5 at lmax 6)
+LARGEST = [("fp64", 0, 1e-12, 1e-12), ("fp32", 1, 1e-5, 1e-4), ("bf16", 2, 1e-1, 2e-1), ("bf16x3", 3, 3e-4, 2e-3)]
+
+
+@pytest.mark.parametrize("name,prec,tol_alpha,tol_beta", LARGEST, ids=[m[0] for m in LARGEST])
+def test_eval_at_the_largest_shape(largest_input, name, prec, tol_alpha, tol_beta):
+    """One normal-equation build at lmax = mmax = 10 in every precision form
+    (the model kernels' widest instance, 8 column blocks of 16, and the
+    normal-equation kernel's 36 tiles) against the oracle.  Start points at
+    three poles and two trial periods, and the same with the pole, period and
+    phase-function parameters perturbed.  (The shape coefficients are not
+    perturbed: the unnormalised Legendre functions reach 6.5e8 at l = m = 10,
+    so small changes to them overflow exp() in curv, in the reference too.)"""
+    from periodsearch_amd import Engine
+
+    eng = Engine(0)
+    inp, orc = _setup(largest_input, eng, prec)
+    assert eng.dims() == (121, 129, 126)
+    nc = 121
+    rng = np.random.default_rng(11)
+    cgs = []
+    for pole in (1, 5, 9):
+        for k in (3, 7):
+            cg1 = orc.initial_cg(inp.freq_start - k * inp.freq_step, pole)
+            cgs.append(cg1)
+            pert = cg1.copy()
+            pert[nc + 1:nc + 7] += rng.normal(0, 1e-3, size=6) * np.maximum(np.abs(cg1[nc + 1:nc + 7]), 0.05)
+            cgs.append(pert)
+    alpha, beta, chi, _ = eng.eval_mrqcof(np.array([c[1:] for c in cgs]), inp.conw_r0)
+    eng.close()
+    worst = [0.0, 0.0, 0.0]
+    for i, cg1 in enumerate(cgs):
+        A, b, c = orc.mrqcof(cg1)
+        assert np.isfinite(A).all() and np.isfinite(b).all() and np.isfinite(c), i
+        ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
+        eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+        ec = float(_rel(chi[i], c))
+        worst = [max(worst[0], ea), max(worst[1], eb), max(worst[2], ec)]
+        assert ea < tol_alpha and eb < tol_beta and ec < (1e-12 if prec == 0 else 1e-5), (i, ea, eb, ec)
+    print(f"lmax 10 {name}: alpha {worst[0]:.2e} beta {worst[1]:.2e} chi2 {worst[2]:.2e} (relative to max)")
