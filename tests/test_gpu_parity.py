@@ -348,3 +348,30 @@ def test_eval_at_the_largest_shape(largest_input, name, prec, tol_alpha, tol_bet
         worst = [max(worst[0], ea), max(worst[1], eb), max(worst[2], ec)]
         assert ea < tol_alpha and eb < tol_beta and ec < (1e-12 if prec == 0 else 1e-5), (i, ea, eb, ec)
     print(f"lmax 10 {name}: alpha {worst[0]:.2e} beta {worst[1]:.2e} chi2 {worst[2]:.2e} (relative to max)")
+
+
+@pytest.mark.parametrize("n_dense,dense_pts,n_sparse,n_lc", [(19, 1000, 1000, 20), (199, 50, 50, 200)],
+                         ids=["20000_points", "200_lightcurves"])
+def test_eval_at_the_input_limits(tmp_path, n_dense, dense_pts, n_sparse, n_lc):
+    """The reference's largest inputs (period_search/constants.h:2-3):
+    MAX_N_OBS = 20,000 points (20 light curves of 1,000, POINTS_MAX), and
+    MAX_LC = 200 light curves (10,000 points): one normal-equation build at
+    six parameter vectors against the oracle, in FP64."""
+    from periodsearch_amd import Engine, synth
+
+    ast = synth.make_asteroid(77, lmax=6, mmax=6, nrows=6, n_dense=n_dense, dense_pts=dense_pts,
+                              n_sparse=n_sparse, noise=0.02, span_years=8, period_h=7.3, incl_deg=8.0)
+    path = str(tmp_path / "period_search_in")
+    synth.write_input(path, ast, 7.2, 7.4)
+    eng = Engine(0)
+    inp, orc = _setup(path, eng)
+    assert orc.iget("Lcurves") == n_lc + 1  # the convexity rows form one more light curve
+    assert orc.iget("ndata") == n_dense * dense_pts + n_sparse + 3
+    cgs = _eval_cases(inp, orc)
+    alpha, beta, chi, _ = eng.eval_mrqcof(np.array([c[1:] for c in cgs]), inp.conw_r0)
+    eng.close()
+    for i, cg1 in enumerate(cgs):
+        A, b, c = orc.mrqcof(cg1)
+        assert np.abs(alpha[i] - A).max() / np.abs(A).max() < 1e-12, i
+        assert np.abs(beta[i] - b).max() / np.abs(b).max() < 1e-12, i
+        assert _rel(chi[i], c) < 1e-12, (i, chi[i], c)
