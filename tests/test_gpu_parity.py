@@ -375,3 +375,67 @@ def test_eval_at_the_input_limits(tmp_path, n_dense, dense_pts, n_sparse, n_lc):
         assert np.abs(alpha[i] - A).max() / np.abs(A).max() < 1e-12, i
         assert np.abs(beta[i] - b).max() / np.abs(b).max() < 1e-12, i
         assert _rel(chi[i], c) < 1e-12, (i, chi[i], c)
+
+
+# Configuration 2 of the mixed-precision study: 5,000 points (20 dense light
+# curves of 150 + 2,000 sparse), lmax = mmax = 8 (81 coefficients), 800
+# facets, 4-12 h.  Bounds from the measured values (in parentheses):
+#   one evaluation vs the oracle, alpha / beta relative to their largest entry
+#     fp32 (3.1e-6 / 8.0e-6), bf16 (2.6e-3 / 0.67), split bf16 (4.2e-6 / 3.6e-4);
+#   the best row of a 41-period window around the true period: the same row in
+#   every form; its chi2 against the FP64 GPU row (which matches the oracle's
+#   ten fits of that period to 2.5e-12): fp32 and split bf16 2.1e-4 lower
+#   (their 50 LM iterations end at a lower point), bf16 1.5e-5.
+C2 = [("fp32", 1, 1e-5, 1e-4, 5e-4), ("bf16", 2, 1e-2, 1.0, 5e-4), ("bf16x3", 3, 1e-4, 2e-3, 5e-4)]
+
+
+@pytest.fixture(scope="module")
+def c2_scan(tmp_path_factory):
+    import oracle
+    from periodsearch_amd import PS_PREC_FP64, Engine, Input, synth
+
+    path = str(tmp_path_factory.mktemp("c2") / "period_search_in")
+    ast = synth.write_workload("big", path, incl_deg=10.0)
+    inp = Input(path)
+    kt = round((inp.freq_start - 1 / ast["period_h"]) / inp.freq_step) + 1
+    orc = oracle.Oracle(path)
+    orc.prepare(inp.conw_r0)
+    eng = Engine(0)
+    eng.set_precision(PS_PREC_FP64)
+    eng.set_problem(inp.problem, keep=inp)
+    rows = eng.scan(inp.freq_start, inp.freq_step, kt - 20, kt + 20, inp.conw_r0)
+    eng.close()
+    return path, inp, orc, kt, rows
+
+
+@pytest.mark.parametrize("name,prec,tol_alpha,tol_beta,tol_chi2", C2, ids=[m[0] for m in C2])
+def test_mixed_forms_at_configuration_2_shape(c2_scan, name, prec, tol_alpha, tol_beta, tol_chi2):
+    from periodsearch_amd import Engine
+
+    path, inp, orc, kt, rows64 = c2_scan
+    nc = orc.iget("Ncoef")
+    assert nc == 81
+    cgs = []
+    rng = np.random.default_rng(7)
+    for pole in (1, 5, 9):
+        cg1 = orc.initial_cg(inp.freq_start - (kt - 1) * inp.freq_step, pole)
+        cgs.append(cg1)
+        pert = cg1.copy()
+        pert[nc + 1:nc + 7] += rng.normal(0, 1e-3, size=6) * np.maximum(np.abs(cg1[nc + 1:nc + 7]), 0.05)
+        cgs.append(pert)
+    eng = Engine(0)
+    eng.set_precision(prec)
+    eng.set_problem(inp.problem, keep=inp)
+    alpha, beta, chi, _ = eng.eval_mrqcof(np.array([c[1:] for c in cgs]), inp.conw_r0)
+    rows = eng.scan(inp.freq_start, inp.freq_step, kt - 20, kt + 20, inp.conw_r0)
+    eng.close()
+    for i, cg1 in enumerate(cgs):
+        A, b, c = orc.mrqcof(cg1)
+        ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
+        eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+        assert ea < tol_alpha and eb < tol_beta and _rel(chi[i], c) < 1e-6, (i, ea, eb)
+    k64, k = int(np.argmin(rows64["chisq"])), int(np.argmin(rows["chisq"]))
+    step_h = rows64["period_h"][k64] ** 2 * inp.freq_step
+    print(f"{name}: best row {k} (FP64 {k64}), chi2 {rows['chisq'][k]:.9g} vs {rows64['chisq'][k64]:.9g}")
+    assert abs(rows["period_h"][k] - rows64["period_h"][k64]) <= step_h
+    assert _rel(rows["chisq"][k], rows64["chisq"][k64]) < tol_chi2
