@@ -4,9 +4,10 @@ The reference starts every trial period from the same ten poles and the
 ellipsoid shape (period_search/period_search_BOINC.cpp:819-844) and keeps the
 best of the ten fits.  This module adds a second way to scan:
 
-  * a network (the GPU Mlp: 66 -> 256 -> 256 -> 12, ReLU, FP32) maps the
+  * a network (the GPU Mlp: 71 -> 256 -> 256 -> 12, ReLU, FP32) maps the
     phase-folded light curve at a trial period (Engine.lc_features: 64 bins,
-    mean solar phase angle, mean |Earth latitude|) to a pole and the l <= 2
+    mean solar phase angle, mean |Earth latitude|) and the asteroid's
+    amplitude-aspect features (aspect_features) to a pole and the l <= 2
     shape coefficients;
   * scan_learned fits every trial period from the proposal (and, if asked,
     its mirror pole): 1 or 2 LM fits instead of 10, each a full LM run under
@@ -41,7 +42,8 @@ import numpy as np
 from . import FIT_DTYPE, ROW_DTYPE, Engine, Input, Mlp, n_coef, synth
 
 NBINS = 64
-N_IN = NBINS + 2
+N_ASPECT = 5
+N_IN = NBINS + 2 + N_ASPECT
 HIDDEN = 256
 N_OUT = 12  # 3 pole + 9 shape coefficients with l <= 2
 DEFAULT_NET = os.path.join(os.path.dirname(os.path.abspath(__file__)), "proposal_net.npz")
@@ -67,6 +69,56 @@ def low_order_index(lmax: int, mmax: int):
     idx = np.array([pos[w] for w in want])
     m1 = np.array([w[1] == 1 for w in want])
     return idx, m1
+
+
+def aspect_features(inp: Input) -> np.ndarray:
+    """Amplitude-aspect features of one asteroid (the same at every trial period).
+
+    A light curve's amplitude grows with the aspect angle between the spin
+    axis and the line of sight.  For views near the ecliptic it varies with
+    the viewing longitude lv as a0 - a1 cos 2(lv - lambda_pole), so the second
+    harmonic of the amplitude against the viewing longitude points at the
+    pole longitude modulo 180 deg (the network's doubled-longitude output).
+    From the dense light curves (at least 10 points within one day): amplitude
+    A = (max - min) / mean brightness and lv = longitude of the mean Earth
+    direction; features [c, s] = sum (A - mean A) (cos 2lv, sin 2lv) /
+    (count * mean A), their length, the mean amplitude, and the count of dense
+    light curves / 10.  Zeros with fewer than two dense light curves."""
+    p = inp.problem
+    n = p.n_points
+    t = np.ctypeslib.as_array(p.tim, shape=(n,))
+    y = np.ctypeslib.as_array(p.brightness, shape=(n,))
+    ee = np.ctypeslib.as_array(p.ee, shape=(3 * n,)).reshape(n, 3)
+    counts = np.ctypeslib.as_array(p.lc_points, shape=(p.n_lc,))
+    amp, lon = [], []
+    q = 0
+    for k in counts:
+        sl = slice(q, q + int(k))
+        q += int(k)
+        if k < 10 or np.ptp(t[sl]) > 1.0:
+            continue
+        m = y[sl].mean()
+        if not m > 0:
+            continue
+        amp.append(np.ptp(y[sl]) / m)
+        e = ee[sl].mean(0)
+        lon.append(np.arctan2(e[1], e[0]))
+    out = np.zeros(N_ASPECT, np.float32)
+    if len(amp) >= 2:
+        a, lv = np.array(amp), np.array(lon)
+        am = a.mean()
+        c = float(np.sum((a - am) * np.cos(2 * lv)) / (len(a) * am))
+        s = float(np.sum((a - am) * np.sin(2 * lv)) / (len(a) * am))
+        out[:] = [c, s, np.hypot(c, s), am, len(a) / 10.0]
+    return out
+
+
+def features(eng: Engine, inp: Input, freq: np.ndarray) -> np.ndarray:
+    """Network inputs at trial frequencies freq: the folded light curve
+    (Engine.lc_features) and the asteroid's amplitude-aspect features."""
+    x = eng.lc_features(freq, NBINS)
+    a = aspect_features(inp)
+    return np.concatenate([x, np.broadcast_to(a, (len(x), N_ASPECT))], axis=1).astype(np.float32)
 
 
 def reference_starts(inp: Input, freq: np.ndarray) -> np.ndarray:
@@ -209,7 +261,7 @@ def scan_learned(eng: Engine, inp: Input, prop: Proposal, conw_r: float, n_from:
         parts = []
         if bi < len(blocks):
             b0, b1 = blocks[bi], min(len(n), blocks[bi] + block)
-            starts = prop.starts(inp, freq[b0:b1], eng.lc_features(freq[b0:b1], NBINS))
+            starts = prop.starts(inp, freq[b0:b1], features(eng, inp, freq[b0:b1]))
             parts.append(starts if mirror else starts[0::2])
         if len(pending):
             parts.append(reference_starts(inp, freq[pending]))
@@ -259,7 +311,7 @@ def samples_for(eng: Engine, cfg: dict, tmpdir: str, n_near: int = 4):
     freq = inp.freq_start - (n - 1) * inp.freq_step
     fits, params = eng.fit_batch_from(reference_starts(inp, freq), inp.conw_r0, want_params=True)
     npol = inp.problem.n_poles
-    x = eng.lc_features(freq, NBINS)
+    x = features(eng, inp, freq)
     t = np.zeros((len(n), N_OUT))
     for i in range(len(n)):
         f = fits[i * npol:(i + 1) * npol]
@@ -361,7 +413,7 @@ def evaluate(prop: Proposal, n_asteroids: int = 16, seed0: int = 5000, window: i
             eng.set_problem(inp2.problem, keep=inp2)
             f = np.array([inp2.freq_start - (n_true - 1) * inp2.freq_step])
             fp = eng.fit_batch_from(reference_starts(inp2, f), inp2.conw_r0)
-            fl = eng.fit_batch_from(prop.starts(inp2, f, eng.lc_features(f, NBINS)), inp2.conw_r0)
+            fl = eng.fit_batch_from(prop.starts(inp2, f, features(eng, inp2, f)), inp2.conw_r0)
             r.update(iter_plain=float(fp["niter"].mean()), iter_learned=float(fl["niter"].mean()),
                      conv_chi2_plain=float(fp["chisq"].min()), conv_chi2_learned=float(fl["chisq"].min()))
             res.append(r)
