@@ -434,8 +434,8 @@ def evaluate(prop: Proposal, n_asteroids: int = 16, seed0: int = 5000, window: i
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="train the proposal network on synthetic asteroids")
-    ap.add_argument("--asteroids", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--asteroids", type=int, default=2200)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--out", default=DEFAULT_NET)
     ap.add_argument("--data", default="", help="also save the training set (.npz)")
     a = ap.parse_args(argv)
