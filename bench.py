@@ -216,6 +216,9 @@ def main() -> int:
                     help="time slot counts and slot groups on this workload first and keep the fastest")
     ap.add_argument("--candidates", type=int, default=16, help="learned form: periods verified per block")
     ap.add_argument("--block", type=int, default=4096, help="learned form: trial periods per block")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the matched-chi2 check after the timed region (the form's best row against the "
+                         "plain FP64 scan's around the true period)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -241,7 +244,7 @@ def main() -> int:
 
     tmp = tempfile.mkdtemp(prefix="psbench_")
     path = os.path.join(tmp, "period_search_in")
-    synth.write_workload(WORKLOAD, path)
+    ast = synth.write_workload(WORKLOAD, path)
     inp = Input(path)
     eng = Engine(0)
     eng.set_precision(prec)
@@ -298,6 +301,38 @@ def main() -> int:
     st = eng.kernel_stats()
     fits = args.steps * P * 10
     value = fits / wall
+
+    def matched_check() -> dict:
+        """Outside the timed region: over the P trial periods around the true
+        period, the best row of this form (for the learned form, from its
+        plain-verified rows) against the best row of the plain FP64 scan."""
+        import numpy as np
+
+        n_true = int(round((inp.freq_start - 1.0 / ast["period_h"]) / inp.freq_step)) + 1
+        n0 = max(1, n_true - P // 2)
+        n1 = min(inp.n_periods, n0 + P - 1)
+        if args.form == "learned":
+            rows, ver, _, _ = scan_learned(eng, inp, prop, conw_r, n0, n1, n_candidates=args.candidates, halo=2,
+                                           block=args.block)
+            k = int(np.flatnonzero(ver)[np.argmin(rows["chisq"][ver])])
+        else:
+            rows = eng.scan(inp.freq_start, inp.freq_step, n0, n1, conw_r)
+            k = int(np.argmin(rows["chisq"]))
+        if prec == PS_PREC_FP64 and args.form != "learned":
+            r64 = rows
+        else:
+            ref = Engine(0)
+            ref.set_precision(PS_PREC_FP64)
+            ref.set_problem(inp.problem, keep=inp)
+            r64 = ref.scan(inp.freq_start, inp.freq_step, n0, n1, conw_r)
+            ref.close()
+        k64 = int(np.argmin(r64["chisq"]))
+        step_h = float(r64["period_h"][k64]) ** 2 * inp.freq_step
+        return {"window": f"trial periods {n0}..{n1} around the true period {ast['period_h']} h",
+                "best_period_h": round(float(rows["period_h"][k]), 8), "chi2": float(rows["chisq"][k]),
+                "fp64_best_period_h": round(float(r64["period_h"][k64]), 8), "fp64_chi2": float(r64["chisq"][k64]),
+                "period_steps_apart": round(abs(float(rows["period_h"][k]) - float(r64["period_h"][k64])) / step_h, 3),
+                "chi2_rel": abs(float(rows["chisq"][k]) - float(r64["chisq"][k64])) / float(r64["chisq"][k64])}
     # algorithmic work of the dominant kernel: the contraction 2*N*F*Ncoef per
     # model evaluation (masked facets counted, as the reference's dense loop)
     flop_eval = 2.0 * N * F * nc
@@ -348,6 +383,8 @@ def main() -> int:
         "job_tflops": round(lm_fits * flop_fit / wall / 1e12, 3),
         "lm_fits_per_s": round(lm_fits / wall, 2),
     }
+    if not args.no_check:
+        out["matched_chi2"] = matched_check()
     if tuned:
         out["config"]["autotuned"] = {"slots": tuned["best"]["slots"], "groups": tuned["best"]["groups"]}
     else:
