@@ -13,7 +13,7 @@ inside it.  The convexity-weight precalc is excluded (it runs identically in
 any implementation).
 
 Forms: --prec fp64 (plain), fp32 / bf16 / bf16x3 (mixed precision), and
---form learned (the default, with FP32: every trial period of the step
+--form learned (the default, with bf16x6: every trial period of the step
 fitted once from the network's start, in each --block-period block the best
 --candidates periods and two neighbours each refitted with the plain ten
 poles; the value is then the plain-equivalent rate, 10 fits per trial period
@@ -51,7 +51,7 @@ WORKLOAD = "wu"
 # profiles/model_mfma_kernel_fp64_wu1080.md and
 # profiles/model_mixed_kernel_bf16x3_learned_wu1080.md; the mixed forms share
 # the FP32 Jacobian layout.
-MODEL_HBM_BYTES_PER_EVAL = {"fp64": 469.4e3, "fp32": 242.2e3, "bf16": 242.2e3, "bf16x3": 242.2e3}
+MODEL_HBM_BYTES_PER_EVAL = {"fp64": 469.4e3, "fp32": 242.2e3, "bf16": 242.2e3, "bf16x3": 242.2e3, "bf16x6": 242.2e3}
 
 
 def cpu_baseline_reference(path: str, conw_r: float, n_procs: int, periods_each: int = 1) -> dict | None:
@@ -210,8 +210,9 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     # default: the fastest form whose best row matches the plain FP64 scan's
-    # chi2 within 1e-4 on this workload (split bf16 is faster but 1.6e-4 off)
-    ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3"), default="fp32")
+    # chi2 within 1e-4 on this workload (three-part split bf16, 1.2e-5; the
+    # two-part split is faster but 1.6e-4 off)
+    ap.add_argument("--prec", choices=("fp64", "fp32", "bf16", "bf16x3", "bf16x6"), default="bf16x6")
     ap.add_argument("--form", choices=("plain", "learned"), default="learned")
     ap.add_argument("--workload", choices=("wu", "batch64"), default="wu",
                     help="batch64: 64 uneven asteroids in one batch of fits (plain start)")
@@ -240,9 +241,11 @@ def main() -> int:
               f"(the trial-period index would shard across GPUs in contiguous ranges)", file=sys.stderr)
         return 2
 
-    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16X3, PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
+    from periodsearch_amd import (PS_PREC_BF16, PS_PREC_BF16X3, PS_PREC_BF16X6, PS_PREC_FP32, PS_PREC_FP64, Engine,
+                                  Input, synth)
 
-    prec = {"fp64": PS_PREC_FP64, "fp32": PS_PREC_FP32, "bf16": PS_PREC_BF16, "bf16x3": PS_PREC_BF16X3}[args.prec]
+    prec = {"fp64": PS_PREC_FP64, "fp32": PS_PREC_FP32, "bf16": PS_PREC_BF16, "bf16x3": PS_PREC_BF16X3,
+            "bf16x6": PS_PREC_BF16X6}[args.prec]
     if args.workload == "batch64":
         return bench_batch(args, prec)
 
@@ -342,7 +345,7 @@ def main() -> int:
     flop_eval = 2.0 * N * F * nc
     achieved = (st["evals"] * flop_eval) / (st["model_ms"] * 1e-3) / 1e12 if st["model_ms"] > 0 else 0.0
     peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS,
-            "bf16x3": BF16_PEAK_TFLOPS}[args.prec]
+            "bf16x3": BF16_PEAK_TFLOPS, "bf16x6": BF16_PEAK_TFLOPS}[args.prec]
     # the model kernel's other half: the facet stage on the vector units (FP64
     # in the plain form, FP32 in the mixed forms), which bounds the mixed forms
     vec_eval = FACET_FLOP_PER_PAIR * N * F
@@ -369,6 +372,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 contraction and alpha (MFMA, f32 accumulate), f64 geometry/chi2/solve",
                   "bf16": "bf16 contraction (MFMA, f32 accumulate), f32 facet stage and alpha, f64 geometry/chi2/solve",
+                  "bf16x6": "three-part split-bf16 contraction (6 MFMA products, f32 accumulate), f32 facet stage "
+                            "and alpha, f64 geometry/chi2/solve",
                   "bf16x3": "split-bf16 contraction (3 MFMA products, f32 accumulate), f32 facet stage and alpha, "
                             "f64 geometry/chi2/solve"}[args.prec],
         "data": "synthetic",
@@ -376,7 +381,7 @@ def main() -> int:
                                "nrows=8 (512 facets), 2-100 h, stop 50, 10 poles",
                    "periods_per_step": P, "fits_per_step": P * 10, "precision": args.prec,
                    "form": {"fp64": "plain FP64", "fp32": "mixed FP32", "bf16": "mixed BF16",
-                            "bf16x3": "mixed split BF16"}[args.prec] + (" + learned proposal" if args.form == "learned"
+                            "bf16x3": "mixed split BF16", "bf16x6": "mixed three-part split BF16"}[args.prec] + (" + learned proposal" if args.form == "learned"
                                                                          else "")},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),

@@ -51,11 +51,14 @@ extern "C" {
  * accumulation on the matrix cores with FP32 accumulation, with FP32
  * operands (PS_PREC_FP32), bf16 operands for the contraction
  * (PS_PREC_BF16; alpha stays FP32) or the contraction operands split into
- * bf16 high and low parts, three bf16 products (PS_PREC_BF16X3). */
+ * bf16 high and low parts, three bf16 products (PS_PREC_BF16X3), or into
+ * three bf16 parts, the six products down to third order (PS_PREC_BF16X6,
+ * about 24 significant bits like FP32). */
 #define PS_PREC_FP64 0
 #define PS_PREC_FP32 1
 #define PS_PREC_BF16 2
 #define PS_PREC_BF16X3 3
+#define PS_PREC_BF16X6 4
 
 typedef struct ps_engine ps_engine;
 

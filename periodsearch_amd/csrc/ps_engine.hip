@@ -97,7 +97,8 @@ struct DevProblem {
   // beyond ncoef and beyond F) in FP32 and in bf16
   const float4 *nor4f;
   const float *dsphT32;
-  const unsigned short *dsphT16, *dsphT16lo;  // bf16 of Dsph, and of Dsph - hi (split form)
+  const unsigned short *dsphT16, *dsphT16lo;  // bf16 of Dsph, and of Dsph - hi (split forms)
+  const unsigned short *dsphT16lo2;           // bf16 of Dsph - hi - lo (three-part split form)
   int fpad;
   const double4 *nor4;    // [F] (nx, ny, nz, 0): one 32-byte scalar load per facet
   const double *dsphP;    // [fpad][16*nb] Dsph zero-padded in rows and columns (FP64 matrix-core kernel)
@@ -1487,7 +1488,7 @@ __global__ void __launch_bounds__(kThreads, NB <= 4 ? 3 : 2) model_mfma_kernel(D
   }
 }
 
-// Mixed-precision model (PS_PREC_FP32, PS_PREC_BF16, PS_PREC_BF16X3).
+// Mixed-precision model (PS_PREC_FP32, PS_PREC_BF16, PS_PREC_BF16X3, PS_PREC_BF16X6).
 // Per-point geometry (phase reduction, rotation matrices, phase function)
 // stays FP64 (point_geometry); the facet stage runs in FP32 and the
 // shape-derivative contraction S = W Dsph on the matrix cores with FP32
@@ -1515,7 +1516,7 @@ __host__ __device__ inline MixedLds mixed_layout(int nfac, int nb, int mode) {
   MixedLds L;
   size_t o = 0;
   const bool bf = mode != 0;
-  const size_t es = bf ? 2 : 4, ws = kFacetChunk + (bf ? 8 : 4), parts = mode == 2 ? 2 : 1;
+  const size_t es = bf ? 2 : 4, ws = kFacetChunk + (bf ? 8 : 4), parts = mode == 3 ? 3 : (mode == 2 ? 2 : 1);
   L.x = o;
   o += kMaxMa * sizeof(double);
   L.blm = o;
@@ -1545,14 +1546,14 @@ __device__ inline unsigned bf16_pair(float a, float b) {
 }
 
 template <int NB, int MODE, bool MULTI>
-__global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) model_mixed_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+__global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4) ? 5 : 4) model_mixed_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
                                                                DevWork W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
   const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
   if (tile >= P.ntiles64) return;
-  constexpr bool BF = MODE != 0, SPLIT = MODE == 2;
+  constexpr bool BF = MODE != 0, SPLIT = MODE >= 2, SPLIT3 = MODE == 3;
   constexpr int NC = 16 * NB;
   constexpr int WS = kFacetChunk + (BF ? 8 : 4);   // padded row, elements (80 / 144 bytes)
   using T = typename std::conditional<BF, unsigned short, float>::type;
@@ -1568,6 +1569,7 @@ __global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) mode
   T *wt = reinterpret_cast<T *>(smb + L.wt);     // [kTilePts][WS] (then the low parts)
   T *dch = reinterpret_cast<T *>(smb + L.dch);   // [NC][WS] (then the low parts)
   T *wtl = wt + kTilePts * WS, *dchl = dch + NC * WS;
+  T *wtl2 = wtl + kTilePts * WS, *dchl2 = dchl + NC * WS;  // third parts (SPLIT3)
   float *red = reinterpret_cast<float *>(smb + L.red);
 
   for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
@@ -1594,13 +1596,14 @@ __global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) mode
   const float4 *__restrict__ n4 = P.nor4f;
   const T *__restrict__ dsT = BF ? reinterpret_cast<const T *>(P.dsphT16) : reinterpret_cast<const T *>(P.dsphT32);
   const T *__restrict__ dsTl = reinterpret_cast<const T *>(P.dsphT16lo);
+  const T *__restrict__ dsTl2 = reinterpret_cast<const T *>(P.dsphT16lo2);
   const int fpad = P.fpad;
   // Dsph^T chunks pass through registers one chunk ahead: the loads of chunk
   // k+1 are in flight during chunk k's facet stage and contraction (fpad is
   // a multiple of the chunk, so a chunk never reads past the table)
   constexpr int kPieces = kFacetChunk / PER16;  // 16-byte pieces per row
   constexpr int kStageIt = (NC * kPieces + kThreads - 1) / kThreads;
-  v4u pre[kStageIt], prel[kStageIt];
+  v4u pre[kStageIt], prel[kStageIt], prel2[SPLIT3 ? kStageIt : 1];
   // (threads past the last piece repeat it: identical loads and stores, no
   // branches, so the pieces stay in registers)
   auto load_chunk = [&](int fc) __attribute__((always_inline)) {
@@ -1610,6 +1613,7 @@ __global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) mode
       const int c = idx / kPieces, pc = idx - c * kPieces;
       pre[it] = *reinterpret_cast<const v4u *>(dsT + (size_t)c * fpad + fc + pc * PER16);
       if (SPLIT) prel[it] = *reinterpret_cast<const v4u *>(dsTl + (size_t)c * fpad + fc + pc * PER16);
+      if constexpr (SPLIT3) prel2[it] = *reinterpret_cast<const v4u *>(dsTl2 + (size_t)c * fpad + fc + pc * PER16);
     });
   };
   load_chunk(0);
@@ -1620,6 +1624,7 @@ __global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) mode
       const int c = idx / kPieces, pc = idx - c * kPieces;
       *reinterpret_cast<v4u *>(dch + c * WS + pc * PER16) = pre[it];
       if (SPLIT) *reinterpret_cast<v4u *>(dchl + c * WS + pc * PER16) = prel[it];
+      if constexpr (SPLIT3) *reinterpret_cast<v4u *>(dchl2 + c * WS + pc * PER16) = prel2[it];
     });
     if (f0 + kFacetChunk < F) load_chunk(f0 + kFacetChunk);
     // facet stage (FP32).  The eight facets of a wave are wave-uniform: their
@@ -1671,12 +1676,18 @@ __global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) mode
       for (int q = 0; q < 4; q++) h[q] = bf16_pair(wv[2 * q], wv[2 * q + 1]);
       *reinterpret_cast<uint4 *>(wt + pl * WS + 8 * sub) = make_uint4(h[0], h[1], h[2], h[3]);
       if constexpr (SPLIT) {
-        unsigned lo[4];
+        unsigned lo[4], lo2[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          lo[q] = bf16_pair(wv[2 * q] - __uint_as_float(h[q] << 16),
-                            wv[2 * q + 1] - __uint_as_float(h[q] & 0xffff0000u));
+        for (int q = 0; q < 4; q++) {
+          const float ra = wv[2 * q] - __uint_as_float(h[q] << 16);
+          const float rb = wv[2 * q + 1] - __uint_as_float(h[q] & 0xffff0000u);
+          lo[q] = bf16_pair(ra, rb);
+          if constexpr (SPLIT3)  // what the low part leaves, exact in FP32
+            lo2[q] = bf16_pair(ra - __uint_as_float(lo[q] << 16), rb - __uint_as_float(lo[q] & 0xffff0000u));
+        }
         *reinterpret_cast<uint4 *>(wtl + pl * WS + 8 * sub) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        if constexpr (SPLIT3)
+          *reinterpret_cast<uint4 *>(wtl2 + pl * WS + 8 * sub) = make_uint4(lo2[0], lo2[1], lo2[2], lo2[3]);
       }
     } else {
       *reinterpret_cast<float4 *>(wt + pl * WS + 8 * sub) = make_float4(wv[0], wv[1], wv[2], wv[3]);
@@ -1687,12 +1698,20 @@ __global__ void __launch_bounds__(kThreads, (MODE != 0 && NB <= 4) ? 5 : 4) mode
     const int ar = 16 * wave + (lane & 15), kg = lane >> 4;
     if constexpr (BF) {
       const v8bf a = *reinterpret_cast<const v8bf *>(wt + ar * WS + 8 * kg);
-      v8bf al;
+      v8bf al, al2;
       if constexpr (SPLIT) al = *reinterpret_cast<const v8bf *>(wtl + ar * WS + 8 * kg);
+      if constexpr (SPLIT3) al2 = *reinterpret_cast<const v8bf *>(wtl2 + ar * WS + 8 * kg);
 #pragma unroll
       for (int cb = 0; cb < NB; cb++) {
         const int br_ = (16 * cb + (lane & 15)) * WS + 8 * kg;
         const v8bf b = *reinterpret_cast<const v8bf *>(dch + br_);
+        if constexpr (SPLIT3) {  // the third-order products first (smallest)
+          const v8bf bl = *reinterpret_cast<const v8bf *>(dchl + br_);
+          const v8bf bl2 = *reinterpret_cast<const v8bf *>(dchl2 + br_);
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al2, b, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl2, acc[cb], 0, 0, 0);
+        }
         if constexpr (SPLIT) {
           const v8bf bl = *reinterpret_cast<const v8bf *>(dchl + br_);
           acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b, acc[cb], 0, 0, 0);
@@ -2187,15 +2206,18 @@ void problem_dims(const ps_problem &p, int &ncoef, int &mfit) {
 int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool allow_valu, KernelSet &ks) {
   const int F = G.nfac, ncoef = G.ncoef, mfit = G.mfit;
   const bool mixed = e->precision != PS_PREC_FP64;
-  const int mode = e->precision == PS_PREC_BF16 ? 1 : (e->precision == PS_PREC_BF16X3 ? 2 : 0);
+  const int mode = e->precision == PS_PREC_BF16 ? 1
+                   : (e->precision == PS_PREC_BF16X3 ? 2 : (e->precision == PS_PREC_BF16X6 ? 3 : 0));
   ks.G = G;
   ks.use_mfma = mixed || !allow_valu;
   if (!mixed)
     ks.model = !ks.use_mfma ? pick_model<false>(G.ncp) : (multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef));
   else if (multi)
-    ks.model = mode == 1 ? pick_mixed<1, true>(ncoef) : (mode == 2 ? pick_mixed<2, true>(ncoef) : pick_mixed<0, true>(ncoef));
+    ks.model = mode == 1 ? pick_mixed<1, true>(ncoef)
+               : (mode == 2 ? pick_mixed<2, true>(ncoef) : (mode == 3 ? pick_mixed<3, true>(ncoef) : pick_mixed<0, true>(ncoef)));
   else
-    ks.model = mode == 1 ? pick_mixed<1, false>(ncoef) : (mode == 2 ? pick_mixed<2, false>(ncoef) : pick_mixed<0, false>(ncoef));
+    ks.model = mode == 1 ? pick_mixed<1, false>(ncoef)
+               : (mode == 2 ? pick_mixed<2, false>(ncoef) : (mode == 3 ? pick_mixed<3, false>(ncoef) : pick_mixed<0, false>(ncoef)));
   if (!ks.model) {
     e->err = "no model kernel for this coefficient count";
     return PS_ERR_LIMIT;
@@ -2586,7 +2608,8 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   std::vector<float4> nor4f(F);
   const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk, ncT = 16 * nb;
   std::vector<float> dsphT32((size_t)ncT * fpad, 0.0f);
-  std::vector<unsigned short> dsphT16((size_t)ncT * fpad, 0), dsphT16lo((size_t)ncT * fpad, 0);
+  std::vector<unsigned short> dsphT16((size_t)ncT * fpad, 0), dsphT16lo((size_t)ncT * fpad, 0),
+      dsphT16lo2((size_t)ncT * fpad, 0);
   std::vector<double> dsphP((size_t)fpad * ncT, 0.0);
   auto to_bf16 = [](float v) {
     unsigned u;
@@ -2608,7 +2631,12 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
       float hf;
       memcpy(&hf, &hb, 4);
       dsphT16[(size_t)c * fpad + f] = hi;
-      dsphT16lo[(size_t)c * fpad + f] = to_bf16(vf - hf);
+      const unsigned short lo = to_bf16(vf - hf);
+      dsphT16lo[(size_t)c * fpad + f] = lo;
+      const unsigned lb = (unsigned)lo << 16;
+      float lf;
+      memcpy(&lf, &lb, 4);
+      dsphT16lo2[(size_t)c * fpad + f] = to_bf16((vf - hf) - lf);
     }
     nor4[f] = make_double4(nx[f], ny[f], nz[f], 0.0);
     nor4f[f] = make_float4((float)nx[f], (float)ny[f], (float)nz[f], 0.0f);
@@ -2652,6 +2680,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   UP(dsphT32.data(), dsphT32.size(), dsphT32);
   UP(dsphT16.data(), dsphT16.size(), dsphT16);
   UP(dsphT16lo.data(), dsphT16lo.size(), dsphT16lo);
+  UP(dsphT16lo2.data(), dsphT16lo2.size(), dsphT16lo2);
   UP(cpleg.data(), cpleg.size(), cpleg);
   UP(cfc.data(), cfc.size(), cfc);
   UP(cfs.data(), cfs.size(), cfs);
@@ -2768,7 +2797,7 @@ const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "nul
 
 int ps_set_precision(ps_engine *e, int precision) {
   if (!e) return PS_ERR_ARG;
-  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16X3) {
+  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16X6) {
     e->err = "unknown precision";
     return PS_ERR_ARG;
   }

@@ -171,7 +171,7 @@ def test_reference_sample_rows(golden_dir):
     engine.close()
 
 
-MIXED = [("fp32", 1, 1e-5, 1e-4), ("bf16", 2, 1e-2, 1e-1), ("bf16x3", 3, 1e-4, 2e-3)]
+MIXED = [("fp32", 1, 1e-5, 1e-4), ("bf16", 2, 1e-2, 1e-1), ("bf16x3", 3, 1e-4, 2e-3), ("bf16x6", 4, 1e-5, 1e-4)]
 MIXED_MIN = [("fp32", 1, 1e-4), ("bf16", 2, 1e-3), ("bf16x3", 3, 1e-4)]
 
 
@@ -309,7 +309,8 @@ def largest_input(tmp_path_factory):
 #   FP32        alpha 1.6e-6  beta 3.4e-6
 #   bf16        alpha 4.2e-2  beta 8.6e-2   (2.8e-3 / 4.1e-2 at lmax 6)
 #   split bf16  alpha 1.1e-4  beta 2.3e-4   (6.3e-6 / 7.9e-5 at lmax 6)
-LARGEST = [("fp64", 0, 1e-12, 1e-12), ("fp32", 1, 1e-5, 1e-4), ("bf16", 2, 1e-1, 2e-1), ("bf16x3", 3, 3e-4, 2e-3)]
+LARGEST = [("fp64", 0, 1e-12, 1e-12), ("fp32", 1, 1e-5, 1e-4), ("bf16", 2, 1e-1, 2e-1), ("bf16x3", 3, 3e-4, 2e-3),
+           ("bf16x6", 4, 1e-5, 1e-4)]
 
 
 @pytest.mark.parametrize("name,prec,tol_alpha,tol_beta", LARGEST, ids=[m[0] for m in LARGEST])
@@ -386,7 +387,8 @@ def test_eval_at_the_input_limits(tmp_path, n_dense, dense_pts, n_sparse, n_lc):
 #   every form; its chi2 against the FP64 GPU row (which matches the oracle's
 #   ten fits of that period to 2.5e-12): fp32 and split bf16 2.1e-4 lower
 #   (their 50 LM iterations end at a lower point), bf16 1.5e-5.
-C2 = [("fp32", 1, 1e-5, 1e-4, 5e-4), ("bf16", 2, 1e-2, 1.0, 5e-4), ("bf16x3", 3, 1e-4, 2e-3, 5e-4)]
+C2 = [("fp32", 1, 1e-5, 1e-4, 5e-4), ("bf16", 2, 1e-2, 1.0, 5e-4), ("bf16x3", 3, 1e-4, 2e-3, 5e-4),
+      ("bf16x6", 4, 1e-5, 1e-4, 5e-4)]
 
 
 @pytest.fixture(scope="module")
