@@ -41,11 +41,11 @@ def _inputs(tmp_path, cases=CASES):
 
 
 @pytest.mark.parametrize("prec_name,cases", [("fp64", "lds"), ("bf16x3", "lds"), ("fp64", "wave"),
-                                              ("bf16x3", "wave")])
+                                              ("bf16x3", "wave"), ("bf16x6", "wave")])
 def test_multi_problem_batch_equals_single_runs(tmp_path, prec_name, cases):
-    from periodsearch_amd import PS_PREC_BF16X3, PS_PREC_FP64, Engine, PsError
+    from periodsearch_amd import PS_PREC_BF16X3, PS_PREC_BF16X6, PS_PREC_FP64, Engine, PsError
 
-    prec = PS_PREC_FP64 if prec_name == "fp64" else PS_PREC_BF16X3
+    prec = {"fp64": PS_PREC_FP64, "bf16x3": PS_PREC_BF16X3, "bf16x6": PS_PREC_BF16X6}[prec_name]
     inps = _inputs(str(tmp_path), CASES if cases == "lds" else CASES_WAVE)
     items = []  # (problem, freq, pole)
     for k, inp in enumerate(inps):
