@@ -165,6 +165,13 @@ int ps_set_groups(ps_engine *e, int groups);
  * in hours, as in the reference driver (period_search_BOINC.cpp:784-786). */
 int ps_precalc(ps_engine *e, double freq_start, double freq_end, double freq_step, double *conw_r,
                int *new_conw);
+/* The same round resumed after its first done_periods periods, whose best
+ * dark-facet % sum to sum_dark_done: the state the optimised CPU app
+ * checkpoints inside a round
+ * (period_search_optimization_cpu/period_search/period_search_BOINC.cpp:627-640, 772-775).
+ * ps_precalc is ps_precalc_resume with 0 and 0.0. */
+int ps_precalc_resume(ps_engine *e, double freq_start, double freq_end, double freq_step, int done_periods,
+                      double sum_dark_done, double *conw_r, int *new_conw);
 
 /* Rows n_from..n_to (1-based, inclusive) of the scan at conw_r (CUDAStart).
  * rows: [n_to-n_from+1].  ndata_minus_3 is taken from the problem. */
@@ -192,6 +199,14 @@ int ps_last_kernel_ms(const ps_engine *e, double *ms, long long *fits);
  * batch).  No counterpart in the reference. */
 int ps_last_batch(const ps_engine *e, double *ms, long long *items, long long *iterations);
 
+/* Progress of a running call (ps_scan, ps_precalc, the fit batches): fn is
+ * called on the calling thread between GPU polls with the fits finished so
+ * far and the fits of the batch; the reference reports the same through
+ * boinc_fraction_done (period_search/period_search_BOINC.cpp:809-810,
+ * period_search_cuda/period_search/start_CUDA.cu:674-678).  fn = NULL turns
+ * it off. */
+typedef void (*ps_progress_fn)(void *user, long long fits_done, long long fits_total);
+int ps_set_progress(ps_engine *e, ps_progress_fn fn, void *user);
 /* Per-launch timing of the model kernel (the dominant kernel) with HIP
  * events on the engine's stream, and a count of model evaluations.
  * ps_set_profiling resets the counters. */
@@ -256,6 +271,46 @@ int ps_read_rows(const char *path, ps_row *rows, long long cap, long long *n);
 int ps_input_read_deferred(const char *path, ps_input **out);
 /* The input's shape request, pointing at its own table storage. */
 int ps_input_shape_job(ps_input *in, ps_shape_job *job);
+
+/* ps_input_read with options:
+ *   PS_READ_DEFER_SHAPE  as ps_input_read_deferred;
+ *   PS_READ_CLAMP_MIN    each time and brightness below DBL_MIN is raised to
+ *                        DBL_MIN as it is read, as the optimised CPU app does
+ *                        (period_search_optimization_cpu/period_search/
+ *                        period_search_BOINC.cpp:371-380); the base app
+ *                        (default) keeps the values as read. */
+#define PS_READ_DEFER_SHAPE 1
+#define PS_READ_CLAMP_MIN 2
+int ps_input_read_ex(const char *path, int flags, ps_input **out);
+
+/* ---- checkpoint (period_search_state) ------------------------------- */
+/* The three reference formats, told apart by their field count:
+ *   PS_STATE_BASE  "nlines"                       period_search/period_search_BOINC.cpp:85-101
+ *   PS_STATE_CUDA  "nlines newConw conw_r"        period_search_cuda/period_search/period_search_BOINC.cpp:96-112
+ *   PS_STATE_OC    "nlines newConw conw_r sum_dark test_periods"
+ *                                                 period_search_optimization_cpu/period_search/period_search_BOINC.cpp:86-101
+ * nlines = rows written; newConw = 1 once the convexity weight is final;
+ * sum_dark / test_periods = the OC app's progress inside a precalc round
+ * (sum of the best dark-facet % over its first test_periods periods). */
+#define PS_STATE_NONE 0
+#define PS_STATE_BASE 1
+#define PS_STATE_CUDA 2
+#define PS_STATE_OC 3
+typedef struct ps_state {
+  int format;       /* PS_STATE_*; NONE = no usable checkpoint (start afresh, truncate the output) */
+  int nlines;
+  int new_conw;
+  double conw_r;
+  double sum_dark;
+  int test_periods;
+} ps_state;
+/* Reads path.  A missing, empty, unparsable or inconsistent file (negative
+ * nlines, non-positive conw_r) gives format PS_STATE_NONE and PS_OK, as the
+ * reference apps fall back to a fresh start (period_search_BOINC.cpp:238-251). */
+int ps_state_read(const char *path, ps_state *st);
+/* Writes st in st->format to path + ".temp" and renames it over path, so a
+ * reader never sees a partial file. */
+int ps_state_write(const char *path, const ps_state *st);
 
 #ifdef __cplusplus
 }

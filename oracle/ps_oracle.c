@@ -61,6 +61,9 @@ struct orc_ctx {
   double freq_start, freq_end, freq_step;
   int new_conw;
   double chck[4];
+  /* dev_new after iteration k of the last fit, k = 1..Niter (test record;
+   * the reference keeps no such record) */
+  double dev_trace[ORC_MAX_N_ITER + 2];
 };
 
 orc_ctx *orc_create(void) { return (orc_ctx *)calloc(1, sizeof(orc_ctx)); }
@@ -1007,6 +1010,7 @@ void orc_fit(orc_ctx *c, double freq, int m, orc_fit_result *out) {
       rchisq = c->Chisq - (pow(c->chck[1], 2) + pow(c->chck[2], 2) + pow(c->chck[3], 2)) * pow(c->conw_r, 2);
     }
     dev_new = sqrt(rchisq / (c->ndata - 3));
+    if (c->Niter <= ORC_MAX_N_ITER) c->dev_trace[c->Niter] = dev_new;
     if (dev_old - dev_new > 1e-10) {
       iter_diff = dev_old - dev_new;
       dev_old = dev_new;
@@ -1128,6 +1132,7 @@ double *orc_dptr(orc_ctx *c, const char *nm) {
   if (!strcmp(nm, "af")) return c->af;
   if (!strcmp(nm, "par")) return c->par;
   if (!strcmp(nm, "chck")) return c->chck;
+  if (!strcmp(nm, "dev_trace")) return c->dev_trace;
   if (!strcmp(nm, "aalpha")) return &c->aalpha[0][0];
   if (!strcmp(nm, "covar")) return &c->covar[0][0];
   return NULL;

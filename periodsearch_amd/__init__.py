@@ -71,6 +71,15 @@ class PsRow(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("period_h", "rms", "chisq", "dark", "lambda_", "beta")]
 
 
+class PsState(C.Structure):
+    _fields_ = [("format", C.c_int), ("nlines", C.c_int), ("new_conw", C.c_int), ("conw_r", C.c_double),
+                ("sum_dark", C.c_double), ("test_periods", C.c_int)]
+
+
+PS_STATE_NONE, PS_STATE_BASE, PS_STATE_CUDA, PS_STATE_OC = 0, 1, 2, 3
+PS_READ_DEFER_SHAPE, PS_READ_CLAMP_MIN = 1, 2
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_longlong, C.c_longlong)
+
 FIT_DTYPE = np.dtype([(n, "f8") for n in ("dev", "chisq", "rchisq", "prd", "la", "be", "dark", "totarea", "alamda")]
                      + [("niter", "i4"), ("err", "i4")])
 ROW_DTYPE = np.dtype([(n, "f8") for n in ("period_h", "rms", "chisq", "dark", "lambda", "beta")])
@@ -104,6 +113,11 @@ def _load() -> C.CDLL:
         "ps_kernel_stats": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ps_input_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
         "ps_input_read_deferred": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "ps_input_read_ex": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
+        "ps_state_read": (C.c_int, [C.c_char_p, C.POINTER(PsState)]),
+        "ps_state_write": (C.c_int, [C.c_char_p, C.POINTER(PsState)]),
+        "ps_precalc_resume": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _ip]),
+        "ps_set_progress": (C.c_int, [vp, PROGRESS_FN, vp]),
         "ps_input_shape_job": (C.c_int, [vp, C.POINTER(PsShapeJob)]),
         "ps_shape_basis_many": (C.c_int, [vp, C.c_int, C.POINTER(PsShapeJob)]),
         "ps_input_free": (None, [vp]),
@@ -141,7 +155,8 @@ EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_
             "ps_precalc", "ps_scan", "ps_fit_batch", "ps_eval_mrqcof", "ps_dims", "ps_last_kernel_ms",
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
             "ps_input_scalars", "ps_format_row", "ps_shape_basis", "ps_shape_basis_many", "ps_input_read_deferred",
-            "ps_input_shape_job", "ps_best_row", "ps_read_rows",
+            "ps_input_shape_job", "ps_best_row", "ps_read_rows", "ps_input_read_ex", "ps_state_read",
+            "ps_state_write", "ps_precalc_resume", "ps_set_progress",
             "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
             "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
             "ps_mlp_train_step")
@@ -224,16 +239,35 @@ def best_period(paths) -> tuple:
     return paths[int(owner[i])], rows[i]
 
 
+def state_read(path: str) -> dict:
+    """A period_search_state checkpoint in any of the three reference formats
+    (ps_state_read); format PS_STATE_NONE when there is none usable."""
+    st = PsState()
+    rc = lib.ps_state_read(os.fsencode(path), C.byref(st))
+    if rc != 0:
+        raise PsError(f"ps_state_read failed: {rc}")
+    return {k: getattr(st, k) for k, _ in PsState._fields_}
+
+
+def state_write(path: str, fmt: int, nlines: int, new_conw: int = 0, conw_r: float = 0.0, sum_dark: float = 0.0,
+                test_periods: int = 0) -> None:
+    st = PsState(fmt, nlines, new_conw, conw_r, sum_dark, test_periods)
+    rc = lib.ps_state_write(os.fsencode(path), C.byref(st))
+    if rc != 0:
+        raise PsError(f"ps_state_write failed: {rc}")
+
+
 class Input:
     """A parsed period_search_in (period_search/period_search_BOINC.cpp:277-485).
 
     With ``defer_shape`` the shape basis is left for ``Engine.shape_basis_inputs``
-    (the GPU shape setup of many inputs at once)."""
+    (the GPU shape setup of many inputs at once); ``clamp_min`` raises times
+    and brightnesses below DBL_MIN to DBL_MIN as the optimised CPU app does."""
 
-    def __init__(self, path: str, defer_shape: bool = False):
+    def __init__(self, path: str, defer_shape: bool = False, clamp_min: bool = False):
         h = C.c_void_p()
-        reader = lib.ps_input_read_deferred if defer_shape else lib.ps_input_read
-        rc = reader(os.fsencode(path), C.byref(h))
+        flags = (PS_READ_DEFER_SHAPE if defer_shape else 0) | (PS_READ_CLAMP_MIN if clamp_min else 0)
+        rc = lib.ps_input_read_ex(os.fsencode(path), flags, C.byref(h))
         if rc != 0:
             raise PsError(f"cannot read {path}: {rc}")
         self._h = h
@@ -336,6 +370,19 @@ class Engine:
         self._check(lib.ps_precalc(self._h, freq_start, freq_end, freq_step, C.byref(cw), C.byref(nc)),
                     "ps_precalc")
         return cw.value, nc.value
+
+    def precalc_resume(self, freq_start: float, freq_end: float, freq_step: float, done_periods: int,
+                       sum_dark: float, conw_r: float):
+        cw = C.c_double(conw_r)
+        nc = C.c_int(0)
+        self._check(lib.ps_precalc_resume(self._h, freq_start, freq_end, freq_step, done_periods, sum_dark,
+                                          C.byref(cw), C.byref(nc)), "ps_precalc_resume")
+        return cw.value, nc.value
+
+    def set_progress(self, fn=None):
+        """fn(fits_done, fits_total) between GPU polls of every call; None turns it off."""
+        self._progress = PROGRESS_FN(lambda _u, d, t: fn(d, t)) if fn else PROGRESS_FN()
+        self._check(lib.ps_set_progress(self._h, self._progress, None), "ps_set_progress")
 
     def eval_mrqcof(self, cg: np.ndarray, conw_r: float):
         cg = np.ascontiguousarray(np.atleast_2d(cg), dtype=np.float64)

@@ -6,7 +6,7 @@ Compiles, for gfx950 only:
                                          (C ABI of include/periodsearch.h and include/ps_proposal.h)
   periodsearch_amd/period_search_amd  -- csrc/ps_main.cpp linked against libpsamd.so
 
-Run ``python -m periodsearch_amd.build`` (or ``__graft_entry__.build()``).
+Run ``python periodsearch_amd/build.py`` (or ``__graft_entry__.build()``).
 """
 from __future__ import annotations
 

@@ -2167,6 +2167,9 @@ struct ps_engine {
   // optional per-launch timing of the model kernel (HIP events on the
   // engine's stream)
   bool profile = false;
+  // progress callback (ps_set_progress), called from the host poll loop
+  ps_progress_fn progress = nullptr;
+  void *progress_user = nullptr;
   std::vector<hipEvent_t> pev;
   double model_ms = 0;
   long long model_launches = 0, evals = 0;
@@ -2438,11 +2441,20 @@ int run_batch(ps_engine *e, DevWork W) {
   if (e->use_graph && !e->profile) {
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(sg[0], hipStreamCaptureModeThreadLocal));
-    HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other lanes' streams into the capture
-    for (int l = 1; l < nl; l++) HIP_TRY(hipStreamWaitEvent(sg[l], e->evf, 0));
-    int rc_cap = enqueue_poll(false);
+    // every call made while capturing is checked, but the capture is always
+    // ended before an error is returned, so stream 0 never stays in capture
+    // mode
+    auto capture_body = [&]() -> int {
+      HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other lanes' streams into the capture
+      for (int l = 1; l < nl; l++) HIP_TRY(hipStreamWaitEvent(sg[l], e->evf, 0));
+      return enqueue_poll(false);
+    };
+    const int rc_cap = capture_body();
     hipError_t err_end = hipStreamEndCapture(sg[0], &graph);
-    if (rc_cap) return rc_cap;
+    if (rc_cap) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc_cap;
+    }
     HIP_TRY(err_end);
     hipError_t err_inst = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -2478,11 +2490,17 @@ int run_batch(ps_engine *e, DevWork W) {
     }
     // host_done: queue, fits done, evaluations (per lane with size classes)
     bool all = true;
+    long long fits_done = 0;
     if (by_class) {
-      for (int l = 0; l < nl; l++) all = all && e->host_done[3 * l + 1] >= lane[l].W.n_items;
+      for (int l = 0; l < nl; l++) {
+        all = all && e->host_done[3 * l + 1] >= lane[l].W.n_items;
+        fits_done += std::min(e->host_done[3 * l + 1], lane[l].W.n_items);
+      }
     } else {
       all = e->host_done[1] >= W.n_items;
+      fits_done = std::min(e->host_done[1], W.n_items);
     }
+    if (e->progress) e->progress(e->progress_user, fits_done, W.n_items);
     if (all) break;
     if (it > guard) {
       e->err = "fit batch did not finish within its iteration bound";
@@ -2938,6 +2956,13 @@ int ps_last_batch(const ps_engine *e, double *ms, long long *items, long long *i
   return PS_OK;
 }
 
+int ps_set_progress(ps_engine *e, ps_progress_fn fn, void *user) {
+  if (!e) return PS_ERR_ARG;
+  e->progress = fn;
+  e->progress_user = user;
+  return PS_OK;
+}
+
 int ps_set_profiling(ps_engine *e, int on) {
   if (!e) return PS_ERR_ARG;
   e->profile = on != 0;
@@ -3017,20 +3042,30 @@ int ps_scan(ps_engine *e, double freq_start, double freq_step, int n_from, int n
 }
 
 int ps_precalc(ps_engine *e, double freq_start, double freq_end, double freq_step, double *conw_r, int *new_conw) {
+  return ps_precalc_resume(e, freq_start, freq_end, freq_step, 0, 0.0, conw_r, new_conw);
+}
+
+int ps_precalc_resume(ps_engine *e, double freq_start, double freq_end, double freq_step, int done_periods,
+                      double sum_dark_done, double *conw_r, int *new_conw) {
   int rc = check_ready(e);
   if (rc) return rc;
-  if (!conw_r || !new_conw) return PS_ERR_ARG;
-  // period_search_BOINC.cpp:571-712
+  if (!conw_r || !new_conw || done_periods < 0) return PS_ERR_ARG;
+  // period_search_BOINC.cpp:571-712; a round resumed after done_periods
+  // periods continues from their sum
+  // (period_search_optimization_cpu/period_search/period_search_BOINC.cpp:627-640, 772)
   int max_test_periods = 10;
   int n_iter = (int)((freq_start - freq_end) / freq_step) + 1;
   if (n_iter < max_test_periods) max_test_periods = n_iter;
-  if (max_test_periods < 1) return PS_ERR_ARG;
-  if ((rc = run_periods(e, freq_start, freq_step, 1, max_test_periods, *conw_r, 1))) return rc;
+  if (max_test_periods < 1 || done_periods > max_test_periods) return PS_ERR_ARG;
   std::vector<ps_row> rows(max_test_periods);
-  HIP_TRY(hipMemcpyAsync(rows.data(), e->d_rows, rows.size() * sizeof(ps_row), hipMemcpyDeviceToHost, e->stream));
-  HIP_TRY(hipStreamSynchronize(e->stream));
-  double sum_dark = 0.0;
-  for (int n = 0; n < max_test_periods; n++) sum_dark = sum_dark + rows[n].dark;
+  if (done_periods < max_test_periods) {
+    if ((rc = run_periods(e, freq_start, freq_step, done_periods + 1, max_test_periods, *conw_r, 1))) return rc;
+    HIP_TRY(hipMemcpyAsync(rows.data(), e->d_rows, (max_test_periods - done_periods) * sizeof(ps_row),
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+  }
+  double sum_dark = sum_dark_done;
+  for (int n = 0; n < max_test_periods - done_periods; n++) sum_dark = sum_dark + rows[n].dark;
   double ave_dark = sum_dark / max_test_periods;
   *new_conw = 0;
   if (ave_dark < 1.0) *new_conw = 1;

@@ -12,6 +12,7 @@
 // bit; the reference build fuses the pair only in the first loop of ellfit,
 // so this file is compiled with -fno-builtin-sin/-cos/-sincos and calls
 // sincos() there explicitly.
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -326,7 +327,7 @@ void psamd_triangulate(int nrows, int *ifp) {
 }
 
 namespace {
-int read_input(const char *path, bool shape, ps_input **out);
+int read_input(const char *path, int flags, ps_input **out);
 }
 
 extern "C" {
@@ -347,9 +348,14 @@ int ps_shape_basis(int nrows, int lmax, int mmax, double a, double b, double c, 
   return nfac;
 }
 
-int ps_input_read(const char *path, ps_input **out) { return read_input(path, true, out); }
+int ps_input_read(const char *path, ps_input **out) { return read_input(path, 0, out); }
 
-int ps_input_read_deferred(const char *path, ps_input **out) { return read_input(path, false, out); }
+int ps_input_read_deferred(const char *path, ps_input **out) { return read_input(path, PS_READ_DEFER_SHAPE, out); }
+
+int ps_input_read_ex(const char *path, int flags, ps_input **out) {
+  if (flags & ~(PS_READ_DEFER_SHAPE | PS_READ_CLAMP_MIN)) return PS_ERR_ARG;
+  return read_input(path, flags, out);
+}
 
 int ps_input_shape_job(ps_input *in, ps_shape_job *job) {
   if (!in || !job) return PS_ERR_ARG;
@@ -375,10 +381,13 @@ namespace {
 
 // period_search_BOINC.cpp:277-485.  Weights use the checked loop of
 // period_search_optimization_cpu/period_search/period_search_BOINC.cpp:440-450.
-// With shape == false the shape tables are sized and left zero, for
-// ps_shape_basis_many to fill.
-int read_input(const char *path, bool shape, ps_input **out) {
+// With PS_READ_DEFER_SHAPE the shape tables are sized and left zero, for
+// ps_shape_basis_many to fill.  PS_READ_CLAMP_MIN raises each time and
+// brightness below DBL_MIN to DBL_MIN as it is read, before the JD range is
+// taken (period_search_optimization_cpu/period_search/period_search_BOINC.cpp:371-380).
+int read_input(const char *path, int flags, ps_input **out) {
   if (!path || !out) return PS_ERR_ARG;
+  const bool shape = !(flags & PS_READ_DEFER_SHAPE), clamp = (flags & PS_READ_CLAMP_MIN) != 0;
   FILE *f = fopen(path, "r");
   if (!f) return PS_ERR_IO;
   auto *in = new ps_input();
@@ -438,6 +447,10 @@ int read_input(const char *path, bool shape, ps_input **out) {
         fclose(f);
         delete in;
         return PS_ERR_IO;
+      }
+      if (clamp) {
+        if (t < DBL_MIN) t = DBL_MIN;
+        if (y < DBL_MIN) y = DBL_MIN;
       }
       if (t < in->jd_min) in->jd_min = t;
       if (t > in->jd_max) in->jd_max = t;
@@ -630,6 +643,56 @@ int ps_format_row(const ps_row *r, int round_pole, char *buf, int buflen) {
   double be = round_pole ? std::round(r->beta) : r->beta;
   return snprintf(buf, buflen, "%.8f  %.6f  %.6f %4.1f %4.0f %4.0f\n", r->period_h, r->rms, r->chisq, r->dark, la,
                   be);
+}
+
+// Checkpoint files.  Each reference app writes one format into a temp file
+// and renames it over period_search_state, and on start accepts only its own
+// field count (anything else starts afresh with the output truncated):
+//   base  "%d"                   period_search/period_search_BOINC.cpp:85-101, 237-251
+//   CUDA  "%d %d %.17g"          period_search_cuda/period_search/period_search_BOINC.cpp:96-112, 287-300
+//   OC    "%d %d %.17g %.17g %d" period_search_optimization_cpu/period_search/period_search_BOINC.cpp:86-101, 238-253
+// ps_state_read accepts all three: the fields read decide the format.
+int ps_state_read(const char *path, ps_state *st) {
+  if (!path || !st) return PS_ERR_ARG;
+  *st = ps_state{};
+  FILE *f = fopen(path, "r");
+  if (!f) return PS_OK;  // no checkpoint: a fresh start
+  int n = fscanf(f, "%d %d %lf %lf %d", &st->nlines, &st->new_conw, &st->conw_r, &st->sum_dark, &st->test_periods);
+  fclose(f);
+  // fields beyond the first complete format are ignored; 2 or 4 fields read
+  // as the shorter format they contain
+  st->format = n >= 5 ? PS_STATE_OC : (n >= 3 ? PS_STATE_CUDA : (n >= 1 ? PS_STATE_BASE : PS_STATE_NONE));
+  if (st->format == PS_STATE_NONE || st->nlines < 0 ||
+      (st->format >= PS_STATE_CUDA && !(st->conw_r > 0 && std::isfinite(st->conw_r))) ||
+      (st->format == PS_STATE_OC && (st->test_periods < 0 || !std::isfinite(st->sum_dark)))) {
+    *st = ps_state{};  // unreadable or inconsistent: a fresh start
+    return PS_OK;
+  }
+  if (st->format < PS_STATE_OC) {
+    st->sum_dark = 0;
+    st->test_periods = 0;
+  }
+  if (st->format < PS_STATE_CUDA) {
+    st->new_conw = 0;
+    st->conw_r = 0;
+  }
+  return PS_OK;
+}
+
+int ps_state_write(const char *path, const ps_state *st) {
+  if (!path || !st || st->format < PS_STATE_BASE || st->format > PS_STATE_OC) return PS_ERR_ARG;
+  const std::string tmp = std::string(path) + ".temp";
+  FILE *f = fopen(tmp.c_str(), "w");
+  if (!f) return PS_ERR_IO;
+  int w;
+  if (st->format == PS_STATE_BASE)
+    w = fprintf(f, "%d", st->nlines);
+  else if (st->format == PS_STATE_CUDA)
+    w = fprintf(f, "%d %d %.17g", st->nlines, st->new_conw, st->conw_r);
+  else
+    w = fprintf(f, "%d %d %.17g %.17g %d", st->nlines, st->new_conw, st->conw_r, st->sum_dark, st->test_periods);
+  if (fclose(f) != 0 || w < 0) return PS_ERR_IO;
+  return rename(tmp.c_str(), path) == 0 ? PS_OK : PS_ERR_IO;
 }
 
 }  // extern "C"

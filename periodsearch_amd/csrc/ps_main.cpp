@@ -3,15 +3,19 @@
 // that runs the fit loop on the GPU through libpsamd.
 //
 //   period_search_amd [--device N] [--in F] [--out F] [--state F] [--batch P]
-//                     [--round-pole] [--prec fp64|fp32]
+//                     [--round-pole] [--prec fp64|fp32|bf16|bf16x3|bf16x6]
+//                     [--state-format base|cuda|oc] [--clamp-min] [--progress]
 //
 // Files default to period_search_in / period_search_out / period_search_state
 // in the working directory (period_search_BOINC.cpp:81-83).  Checkpoints are
 // written atomically (temp file + rename, period_search_BOINC.cpp:85-101) in
-// the CUDA app's format "nlines newConw conw_r"
-// (period_search_cuda/period_search/period_search_BOINC.cpp:96-112); on start
-// the base ("nlines"), CUDA and optimised-CPU ("nlines newConw conw_r sum
-// testperiods") formats are all accepted.
+// one of the three reference formats (--state-format, default the CUDA app's
+// "nlines newConw conw_r", period_search_cuda/period_search/period_search_BOINC.cpp:96-112);
+// on start any of the three is accepted (ps_state_read), and a missing or
+// unreadable checkpoint starts afresh with the output truncated
+// (period_search_BOINC.cpp:238-251).  An optimised-CPU-app checkpoint taken
+// inside a convexity-weight round resumes that round
+// (period_search_optimization_cpu/period_search/period_search_BOINC.cpp:627-640).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,61 +26,79 @@
 
 namespace {
 
-struct State {
-  int nlines = 0, new_conw = 0;
-  double conw_r = 0;
-  int fields = 0;
-};
-
-State read_state(const std::string &path) {
-  State s;
-  FILE *f = fopen(path.c_str(), "r");
-  if (!f) return s;
-  s.fields = fscanf(f, "%d %d %lf", &s.nlines, &s.new_conw, &s.conw_r);
-  fclose(f);
-  if (s.fields < 1) s.fields = 0;
-  return s;
+void print_progress(void *, long long done, long long total) {
+  fprintf(stderr, "  fits %lld of %lld\n", done, total);
 }
 
-int write_state(const std::string &path, int nlines, int new_conw, double conw_r) {
-  std::string tmp = path + ".temp";
-  FILE *f = fopen(tmp.c_str(), "w");
-  if (!f) return 1;
-  fprintf(f, "%d %d %.17g", nlines, new_conw, conw_r);
-  fclose(f);
-  return rename(tmp.c_str(), path.c_str());
+int parse_prec(const std::string &s) {
+  if (s == "fp64") return PS_PREC_FP64;
+  if (s == "fp32") return PS_PREC_FP32;
+  if (s == "bf16") return PS_PREC_BF16;
+  if (s == "bf16x3") return PS_PREC_BF16X3;
+  if (s == "bf16x6") return PS_PREC_BF16X6;
+  return -1;
+}
+
+int parse_state_format(const std::string &s) {
+  if (s == "base") return PS_STATE_BASE;
+  if (s == "cuda") return PS_STATE_CUDA;
+  if (s == "oc") return PS_STATE_OC;
+  return -1;
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-  int device = 0, batch = 512, round_pole = 0, prec = PS_PREC_FP64;
+  int device = 0, batch = 512, round_pole = 0, prec = PS_PREC_FP64, state_format = PS_STATE_CUDA, flags = 0;
+  bool progress = false;
   std::string in_path = "period_search_in", out_path = "period_search_out", state_path = "period_search_state";
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
-    auto next = [&]() -> const char * { return i + 1 < argc ? argv[++i] : ""; };
-    if (a == "--device")
-      device = atoi(next());
-    else if (a == "--in")
+    auto next = [&]() -> std::string {
+      if (i + 1 >= argc) {
+        fprintf(stderr, "%s needs a value\n", a.c_str());
+        exit(2);
+      }
+      return argv[++i];
+    };
+    if (a == "--device") {
+      device = atoi(next().c_str());
+    } else if (a == "--in") {
       in_path = next();
-    else if (a == "--out")
+    } else if (a == "--out") {
       out_path = next();
-    else if (a == "--state")
+    } else if (a == "--state") {
       state_path = next();
-    else if (a == "--batch")
-      batch = atoi(next());
-    else if (a == "--round-pole")
+    } else if (a == "--batch") {
+      batch = atoi(next().c_str());
+    } else if (a == "--round-pole") {
       round_pole = 1;
-    else if (a == "--prec")
-      prec = std::string(next()) == "fp32" ? PS_PREC_FP32 : PS_PREC_FP64;
-    else {
+    } else if (a == "--clamp-min") {
+      flags |= PS_READ_CLAMP_MIN;
+    } else if (a == "--progress") {
+      progress = true;
+    } else if (a == "--prec") {
+      const std::string v = next();
+      prec = parse_prec(v);
+      if (prec < 0) {
+        fprintf(stderr, "unknown precision '%s' (fp64, fp32, bf16, bf16x3, bf16x6)\n", v.c_str());
+        return 2;
+      }
+    } else if (a == "--state-format") {
+      const std::string v = next();
+      state_format = parse_state_format(v);
+      if (state_format < 0) {
+        fprintf(stderr, "unknown checkpoint format '%s' (base, cuda, oc)\n", v.c_str());
+        return 2;
+      }
+    } else {
       fprintf(stderr, "unknown argument %s\n", a.c_str());
       return 2;
     }
   }
   if (batch < 1) batch = 1;
   ps_input *in = nullptr;
-  if (ps_input_read(in_path.c_str(), &in) != PS_OK) {
+  if (ps_input_read_ex(in_path.c_str(), flags, &in) != PS_OK) {
     fprintf(stderr, "Couldn't read input file %s\n", in_path.c_str());
     return 1;
   }
@@ -97,30 +119,48 @@ int main(int argc, char **argv) {
     fprintf(stderr, "engine setup failed: %s\n", ps_last_error(eng));
     return 2;
   }
-  State st = read_state(state_path);
-  const bool resume = st.fields >= 1;
-  int n_start = resume ? st.nlines + 1 : 1;
+  if (progress) ps_set_progress(eng, print_progress, nullptr);
+  ps_state st;
+  ps_state_read(state_path.c_str(), &st);
+  const bool resume = st.format != PS_STATE_NONE;
+  const int n_start = resume ? st.nlines + 1 : 1;
+  // The convexity weight: the base format does not carry it, so the precalc
+  // runs again (period_search/period_search_BOINC.cpp:487); the CUDA and OC
+  // formats carry it, and newConw = 1 marks it final.
   double conw_r = conw / escl / escl;
-  int new_conw = 0;
-  if (resume && st.fields >= 3) {
+  int new_conw = 0, done_periods = 0;
+  double sum_dark = 0;
+  if (st.format >= PS_STATE_CUDA) {
     conw_r = st.conw_r;
     new_conw = st.new_conw;
-  }
-  // convexity-weight precalc (period_search_BOINC.cpp:487-715); skipped when
-  // the checkpoint carries the converged weight.
-  if (!(resume && st.fields >= 3 && new_conw == 1)) {
-    new_conw = 0;
-    if (!(resume && st.fields >= 3)) conw_r = conw / escl / escl;
-    while (new_conw != 1 && conw_r * escl * escl < 10.0) {
-      if (ps_precalc(eng, freq_start, freq_end, freq_step, &conw_r, &new_conw) != PS_OK) {
-        fprintf(stderr, "precalc failed: %s\n", ps_last_error(eng));
-        return 2;
-      }
-      if (write_state(state_path, n_start - 1, new_conw, conw_r)) {
-        fprintf(stderr, "checkpoint failed\n");
-        return 1;
-      }
+    if (st.format == PS_STATE_OC && new_conw != 1) {
+      done_periods = st.test_periods;
+      sum_dark = st.sum_dark;
     }
+  }
+  auto checkpoint = [&](int nlines, int sum_periods) {
+    ps_state w{};
+    w.format = state_format;
+    w.nlines = nlines;
+    w.new_conw = new_conw;
+    w.conw_r = conw_r;
+    w.sum_dark = sum_periods ? sum_dark : 0.0;
+    w.test_periods = sum_periods;
+    if (ps_state_write(state_path.c_str(), &w) != PS_OK) {
+      fprintf(stderr, "checkpoint failed\n");
+      exit(1);
+    }
+  };
+  // convexity-weight precalc (period_search_BOINC.cpp:487-715)
+  while (new_conw != 1 && conw_r * escl * escl < 10.0) {
+    if (ps_precalc_resume(eng, freq_start, freq_end, freq_step, done_periods, sum_dark, &conw_r, &new_conw) !=
+        PS_OK) {
+      fprintf(stderr, "precalc failed: %s\n", ps_last_error(eng));
+      return 2;
+    }
+    done_periods = 0;
+    sum_dark = 0;
+    checkpoint(n_start - 1, 0);
   }
   FILE *out = fopen(out_path.c_str(), n_start > 1 ? "a" : "w");
   if (!out) {
@@ -143,10 +183,8 @@ int main(int argc, char **argv) {
       fputs(buf, out);
     }
     fflush(out);
-    if (write_state(state_path, n1, 1, conw_r)) {
-      fprintf(stderr, "checkpoint failed\n");
-      return 1;
-    }
+    new_conw = 1;
+    checkpoint(n1, 0);
     fprintf(stderr, "rows %d..%d of %d done\n", n0, n1, n_periods);
   }
   fclose(out);

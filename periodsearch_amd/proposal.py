@@ -210,14 +210,26 @@ def _rows_from_fits(fits: np.ndarray, per: int, n_points: int) -> np.ndarray:
     the first fit with the smallest rms, lambda wrapped to [0, 360),
     chi2 = rms^2 (ndata - 3) with ndata = n_points + the 3 convexity rows)."""
     f = fits.reshape(-1, per)
-    b = f[np.arange(len(f)), np.argmin(f["dev"], axis=1)]
+    # the reference keeps the first fit whose dev is strictly below the best
+    # so far, starting from 1e40: a NaN dev (a diverged fit) never wins, and
+    # if no fit qualifies the row keeps dev 1e40 and zeros
+    # (period_search_BOINC.cpp:913-924)
+    dev = f["dev"]
+    ok = dev < 1e40  # False for NaN
+    key = np.where(ok, dev, np.inf)
+    k = np.argmin(key, axis=1)
+    any_ok = ok.any(axis=1)
+    b = f[np.arange(len(f)), k]
     rows = np.zeros(len(f), dtype=ROW_DTYPE)
-    rows["period_h"] = 24 * b["prd"]
-    rows["rms"] = b["dev"]
-    rows["chisq"] = b["dev"] ** 2 * n_points
-    rows["dark"] = b["dark"] / b["totarea"] * 100
-    rows["lambda"] = np.where(b["la"] < 0, b["la"] + 360, b["la"])
-    rows["beta"] = b["be"]
+    dev_b = np.where(any_ok, b["dev"], 1e40)
+    rows["period_h"] = np.where(any_ok, 24 * b["prd"], 0.0)
+    rows["rms"] = dev_b
+    rows["chisq"] = dev_b ** 2 * n_points
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rows["dark"] = np.where(any_ok, b["dark"] / b["totarea"] * 100, 0.0)
+    la = np.where(any_ok, b["la"], 0.0)
+    rows["lambda"] = np.where(la < 0, la + 360, la)
+    rows["beta"] = np.where(any_ok, b["be"], 0.0)
     return rows
 
 
