@@ -212,6 +212,11 @@ int ps_set_progress(ps_engine *e, ps_progress_fn fn, void *user);
  * ps_set_profiling resets the counters. */
 int ps_set_profiling(ps_engine *e, int on);
 int ps_kernel_stats(const ps_engine *e, double *model_ms, long long *model_launches, long long *evals);
+/* How the model kernel was timed since ps_set_profiling: 1 = events captured
+ * as nodes of the poll graph (the default path, as run without profiling),
+ * 2 = plain launches with events between them (PSAMD_GRAPH=0, or a runtime
+ * that cannot time graph events), 0 = nothing timed yet. */
+int ps_kernel_timing_mode(const ps_engine *e);
 
 /* ---- host-side driver helpers ---------------------------------------- */
 /* Parsed and normalised period_search_in (period_search_BOINC.cpp:277-485). */
@@ -271,6 +276,14 @@ int ps_read_rows(const char *path, ps_row *rows, long long cap, long long *n);
 int ps_input_read_deferred(const char *path, ps_input **out);
 /* The input's shape request, pointing at its own table storage. */
 int ps_input_shape_job(ps_input *in, ps_shape_job *job);
+
+/* The best pole of each trial period (period_search_BOINC.cpp:900-924) over
+ * fits from ps_fit_batch-like calls: per consecutive fits per period, the
+ * first fit whose dev is strictly below the best so far (starting at 1e40)
+ * wins, so a NaN dev never does and a period with none keeps dev 1e40 and
+ * zeros; dark = |chck| / totarea * 100; lambda wrapped to [0, 360); chi2 =
+ * dev^2 * n_points (n_points = ndata - 3, the real points). */
+int ps_rows_from_fits(const ps_fit_out *fits, int n_periods, int per, int n_points, ps_row *rows);
 
 /* ps_input_read with options:
  *   PS_READ_DEFER_SHAPE  as ps_input_read_deferred;

@@ -57,6 +57,53 @@ int ps_mlp_gradients(ps_mlp *m, const float *x, const float *t, const float *w_o
 int ps_mlp_train_step(ps_mlp *m, const float *x, const float *t, const float *w_out, int batch, float lr,
                       float *loss);
 
+/* Amplitude-aspect features of one asteroid, the same at every trial
+ * period: from the dense light curves (at least 10 points within one day),
+ * amplitude A = (max - min) / mean brightness and the longitude lv of the
+ * mean Earth direction; out = [c, s, hypot(c, s), mean A, count / 10] with
+ * (c, s) = sum (A - mean A) (cos 2 lv, sin 2 lv) / (count * mean A) -- the
+ * second harmonic of amplitude against viewing longitude, which points at
+ * the pole longitude modulo 180 deg.  Zeros with fewer than two such light
+ * curves. */
+int ps_aspect_features(const ps_problem *p, float out[5]);
+
+/* A trained proposal network: n_in = nbins + 2 + 5 inputs (ps_lc_features
+ * then ps_aspect_features), scaled as (x - x_mean) / x_std; n_out = 12
+ * outputs, unscaled as y * t_std + t_mean: the doubled-longitude pole
+ * (cos b cos 2l, cos b sin 2l, sin b) and the 9 coefficients with l <= 2
+ * (c00, c10, c20, c11 cos/sin, c21 cos/sin, c22 cos/sin; c00 is not used). */
+typedef struct ps_proposal_net {
+  ps_mlp *mlp;
+  int n_in, n_out, nbins, max_batch;  /* max_batch: the value given to ps_mlp_create */
+  const float *x_mean, *x_std, *t_mean, *t_std;
+} ps_proposal_net;
+
+/* How the learned scan screens and judges. */
+typedef struct ps_learned_opts {
+  int block;         /* trial periods per screening batch */
+  int n_candidates;  /* lowest-rms periods per block sent to the judge (no two within halo) */
+  int halo;          /* neighbours on either side of each candidate also judged */
+  int mirror;        /* 1: fit the mirror pole (lambda + 180 deg) as well */
+  ps_engine *judge;  /* engine for the plain ten-pole judge fits (its precision, e.g. FP64, and the same
+                      * problem); NULL: the screening engine.  Another engine runs a block's judge fits on
+                      * its own thread while the next block is screened. */
+} ps_learned_opts;
+
+/* Rows n_from..n_to of a learned scan (no counterpart in the reference, whose
+ * every trial period is fitted from ten poles, period_search_BOINC.cpp:807-924).
+ * Each trial period is fitted by the full LM loop (the problem's stop rule)
+ * from the network's proposed start (the reference's start with the proposed
+ * pole and l <= 2 shape); per block the candidates and their neighbours are
+ * refitted from the reference's ten poles by the judge, and those rows
+ * replace the screened ones: judged[i] = 1 marks them, the rows in the
+ * reference's sense.  The other rows are the screening fits' (period of the
+ * best screening fit, its rms and chi2).  p is the problem given to
+ * ps_set_problem of e (and of the judge).  screened (optional, [n]) receives
+ * every period's screening row; *n_fits the LM fits run (screening + judge). */
+int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *net, const ps_learned_opts *opt,
+                    double freq_start, double freq_step, int n_from, int n_to, double conw_r, ps_row *rows,
+                    int *judged, ps_row *screened, long long *n_fits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,11 +9,13 @@ the round and builds the shape basis; its tables are copied into the
 reference's globals (oracle.ref_load_problem).  tests/test_oracle_ref.py shows
 that these tables and whole fits are bit-identical between the two.
 
-usage: python -m oracle.ref_bench <period_search_in> <conw_r> <n_from> <n_to>
+usage: python -m oracle.ref_bench <period_search_in> <conw_r> <n_from> <n_to> [n_poles]
 
 For trial periods n_from..n_to (1-based, the reference's index n,
 period_search_BOINC.cpp:807-812), runs the ten-pole fits
-(period_search_BOINC.cpp:197-206, 819-845) in the reference's order.  Prints
+(period_search_BOINC.cpp:197-206, 819-845) in the reference's order, or the
+first n_poles of them (a bounded sample where one fit takes many seconds:
+under a stop condition > 1 every fit of an asteroid costs the same).  Prints
 one JSON line: fits run, and the wall clock (time.time()) at the first fit's
 start and after the last fit's end.
 """
@@ -33,6 +35,7 @@ POLE_LAMBDA = [0, 90, 180, 270, 60, 180, 300, 60, 180, 300]
 
 def main(argv) -> int:
     path, conw_r, n_from, n_to = argv[0], float(argv[1]), int(argv[2]), int(argv[3])
+    n_poles = int(argv[4]) if len(argv) > 4 else 10
     ref = oracle.load_ref()
     if ref is None:
         print(json.dumps({"error": "oracle/_ref not built"}), flush=True)
@@ -51,7 +54,7 @@ def main(argv) -> int:
     t0 = time.time()
     for n in range(n_from, n_to + 1):
         freq = f0 - (n - 1) * df
-        for m in range(10):
+        for m in range(n_poles):
             ref.ref_fit(freq, POLE_BETA[m], POLE_LAMBDA[m], oracle._d(par), oracle._i(iap), cl, stop, nmin, conw_r, r)
             fits += 1
     t1 = time.time()

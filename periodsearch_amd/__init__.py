@@ -71,6 +71,16 @@ class PsRow(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("period_h", "rms", "chisq", "dark", "lambda_", "beta")]
 
 
+class PsProposalNet(C.Structure):
+    _fields_ = [("mlp", C.c_void_p), ("n_in", C.c_int), ("n_out", C.c_int), ("nbins", C.c_int),
+                ("max_batch", C.c_int), ("x_mean", _fp), ("x_std", _fp), ("t_mean", _fp), ("t_std", _fp)]
+
+
+class PsLearnedOpts(C.Structure):
+    _fields_ = [("block", C.c_int), ("n_candidates", C.c_int), ("halo", C.c_int), ("mirror", C.c_int),
+                ("judge", C.c_void_p)]
+
+
 class PsState(C.Structure):
     _fields_ = [("format", C.c_int), ("nlines", C.c_int), ("new_conw", C.c_int), ("conw_r", C.c_double),
                 ("sum_dark", C.c_double), ("test_periods", C.c_int)]
@@ -118,6 +128,12 @@ def _load() -> C.CDLL:
         "ps_state_write": (C.c_int, [C.c_char_p, C.POINTER(PsState)]),
         "ps_precalc_resume": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _dp, _ip]),
         "ps_set_progress": (C.c_int, [vp, PROGRESS_FN, vp]),
+        "ps_aspect_features": (C.c_int, [C.POINTER(PsProblem), _fp]),
+        "ps_rows_from_fits": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+        "ps_kernel_timing_mode": (C.c_int, [vp]),
+        "ps_scan_learned": (C.c_int, [vp, C.POINTER(PsProblem), C.POINTER(PsProposalNet), C.POINTER(PsLearnedOpts),
+                                      C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, vp, _ip, vp,
+                                      C.POINTER(C.c_longlong)]),
         "ps_input_shape_job": (C.c_int, [vp, C.POINTER(PsShapeJob)]),
         "ps_shape_basis_many": (C.c_int, [vp, C.c_int, C.POINTER(PsShapeJob)]),
         "ps_input_free": (None, [vp]),
@@ -156,7 +172,8 @@ EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_
             "ps_set_profiling", "ps_kernel_stats", "ps_input_read", "ps_input_free", "ps_input_problem",
             "ps_input_scalars", "ps_format_row", "ps_shape_basis", "ps_shape_basis_many", "ps_input_read_deferred",
             "ps_input_shape_job", "ps_best_row", "ps_read_rows", "ps_input_read_ex", "ps_state_read",
-            "ps_state_write", "ps_precalc_resume", "ps_set_progress",
+            "ps_state_write", "ps_precalc_resume", "ps_set_progress", "ps_aspect_features", "ps_scan_learned",
+            "ps_rows_from_fits", "ps_kernel_timing_mode",
             "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
             "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
             "ps_mlp_train_step")
@@ -206,6 +223,18 @@ def _shape_arrays(nrows: int, lmax: int, mmax: int) -> dict:
     return dict(nor=np.zeros(F * 3), darea=np.zeros(F), pleg=np.zeros(F * (lmax + 1) ** 2),
                 fc=np.zeros(F * (mmax + 1)), fs=np.zeros(F * (mmax + 1)), dsph=np.zeros(F * n_coef(lmax, mmax)),
                 cg_first=np.zeros(n_coef(lmax, mmax)))
+
+
+def rows_from_fits(fits: np.ndarray, per: int, n_points: int) -> np.ndarray:
+    """Best-of-`per` rows of consecutive fits (ps_rows_from_fits: the
+    reference's pole reduction)."""
+    fits = np.ascontiguousarray(fits, dtype=FIT_DTYPE)
+    n = len(fits) // per
+    rows = np.zeros(n, dtype=ROW_DTYPE)
+    rc = lib.ps_rows_from_fits(fits.ctypes.data, n, per, n_points, rows.ctypes.data)
+    if rc != 0:
+        raise PsError(f"ps_rows_from_fits failed: {rc}")
+    return rows
 
 
 def read_rows(path: str, cap: int = 1 << 20) -> np.ndarray:
@@ -425,7 +454,8 @@ class Engine:
         n = C.c_longlong()
         ev = C.c_longlong()
         lib.ps_kernel_stats(self._h, C.byref(ms), C.byref(n), C.byref(ev))
-        return dict(model_ms=ms.value, model_launches=n.value, evals=ev.value)
+        mode = {1: "graph event nodes", 2: "plain launches"}.get(lib.ps_kernel_timing_mode(self._h), "none")
+        return dict(model_ms=ms.value, model_launches=n.value, evals=ev.value, timing=mode)
 
     def last_batch(self):
         """(GPU ms, fits, LM iterations) of the last batch of fits."""

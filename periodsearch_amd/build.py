@@ -2,7 +2,7 @@
 
 Compiles, for gfx950 only:
   periodsearch_amd/libpsamd.so        -- csrc/ps_engine.hip + csrc/ps_mlp.hip + csrc/ps_shape.hip
-                                         + csrc/ps_host.cpp
+                                         + csrc/ps_host.cpp + csrc/ps_learned.cpp
                                          (C ABI of include/periodsearch.h and include/ps_proposal.h)
   periodsearch_amd/period_search_amd  -- csrc/ps_main.cpp linked against libpsamd.so
 
@@ -59,10 +59,13 @@ def build(verbose: bool = False, defines=(), lib_out: str | None = None, driver:
     _run(["g++", "-O3", "-fPIC", "-std=c++17", "-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sincos",
           "-I", INCLUDE, "-c", "-o", host_o,
           os.path.join(CSRC, "ps_host.cpp")])
-    objs = [eng_o, mlp_o, shape_o, host_o]
+    learned_o = os.path.join(HERE, "_obj", "ps_learned.o")
+    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-pthread", "-I", INCLUDE, "-c", "-o", learned_o,
+          os.path.join(CSRC, "ps_learned.cpp")])
+    objs = [eng_o, mlp_o, shape_o, host_o, learned_o]
     lib = lib_out or LIB
     tmp = lib + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, lib)
     if not driver:
         return lib

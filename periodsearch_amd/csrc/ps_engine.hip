@@ -2167,6 +2167,8 @@ struct ps_engine {
   // optional per-launch timing of the model kernel (HIP events on the
   // engine's stream)
   bool profile = false;
+  bool graph_events_ok = true;  // timing events can be captured into the poll graph
+  int timing_mode = 0;          // how model_ms was measured: 1 graph event nodes, 2 plain launches
   // progress callback (ps_set_progress), called from the host poll loop
   ps_progress_fn progress = nullptr;
   void *progress_user = nullptr;
@@ -2438,7 +2440,11 @@ int run_batch(ps_engine *e, DevWork W) {
     return PS_OK;
   };
   hipGraphExec_t gexec = nullptr;
-  if (e->use_graph && !e->profile) {
+  // With profiling the timing events are captured into the graph as event
+  // record nodes, so the timed path is the graph path; if the runtime cannot
+  // capture or time them, profiling falls back to plain launches with events
+  // between them (e->graph_events_ok false from then on).
+  if (e->use_graph && (!e->profile || e->graph_events_ok)) {
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(sg[0], hipStreamCaptureModeThreadLocal));
     // every call made while capturing is checked, but the capture is always
@@ -2447,18 +2453,24 @@ int run_batch(ps_engine *e, DevWork W) {
     auto capture_body = [&]() -> int {
       HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other lanes' streams into the capture
       for (int l = 1; l < nl; l++) HIP_TRY(hipStreamWaitEvent(sg[l], e->evf, 0));
-      return enqueue_poll(false);
+      return enqueue_poll(e->profile);
     };
     const int rc_cap = capture_body();
     hipError_t err_end = hipStreamEndCapture(sg[0], &graph);
-    if (rc_cap) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc_cap;
+    hipError_t err_inst = hipSuccess;
+    if (!rc_cap && err_end == hipSuccess) err_inst = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc_cap || err_end != hipSuccess || err_inst != hipSuccess) {
+      gexec = nullptr;
+      if (!e->profile) {
+        if (rc_cap) return rc_cap;
+        HIP_TRY(err_end);
+        HIP_TRY(err_inst);
+      }
+      (void)hipGetLastError();
+      e->graph_events_ok = false;
+      e->err.clear();
     }
-    HIP_TRY(err_end);
-    hipError_t err_inst = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HIP_TRY(err_inst);
   }
   struct GraphGuard {
     hipGraphExec_t &g;
@@ -2480,13 +2492,26 @@ int run_batch(ps_engine *e, DevWork W) {
     HIP_TRY(hipMemcpyAsync(e->host_done, e->counters, ncnt * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->profile) {
-      for (int k = 0; k < poll; k++)
-        for (int l = 0; l < nl; l++) {
+      double sum = 0;
+      bool ok = true;
+      for (int k = 0; k < poll && ok; k++)
+        for (int l = 0; l < nl && ok; l++) {
           float ms = 0;
-          HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + l)], e->pev[2 * (kMaxGroups * k + l) + 1]));
-          e->model_ms += ms;
+          ok = hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + l)], e->pev[2 * (kMaxGroups * k + l) + 1]) ==
+               hipSuccess;
+          sum += ms;
         }
-      e->model_launches += poll * nl;
+      if (ok) {
+        e->model_ms += sum;
+        e->model_launches += poll * nl;
+        e->timing_mode = gexec ? 1 : 2;
+      } else if (gexec) {
+        (void)hipGetLastError();  // event nodes not timed by this runtime: plain launches from the next batch
+        e->graph_events_ok = false;
+      } else {
+        e->err = "timing the model kernel with HIP events failed";
+        return PS_ERR_HIP;
+      }
     }
     // host_done: queue, fits done, evaluations (per lane with size classes)
     bool all = true;
@@ -2966,11 +2991,14 @@ int ps_set_progress(ps_engine *e, ps_progress_fn fn, void *user) {
 int ps_set_profiling(ps_engine *e, int on) {
   if (!e) return PS_ERR_ARG;
   e->profile = on != 0;
+  e->timing_mode = 0;
   e->model_ms = 0;
   e->model_launches = 0;
   e->evals = 0;
   return PS_OK;
 }
+
+int ps_kernel_timing_mode(const ps_engine *e) { return e ? e->timing_mode : PS_ERR_ARG; }
 
 int ps_kernel_stats(const ps_engine *e, double *model_ms, long long *model_launches, long long *evals) {
   if (!e) return PS_ERR_ARG;

@@ -1,0 +1,334 @@
+// ps_learned.cpp -- the learned scan: trial periods started from a
+// network's proposal, screened, and the best candidates judged by the plain
+// ten-pole fits.  Host orchestration only: the fits, the light-curve
+// features and the network run in the engine's and the network's HIP kernels
+// (ps_engine.hip, ps_mlp.hip) through their C ABI.
+//
+// The reference starts every trial period from the same ten poles and the
+// ellipsoid shape (period_search/period_search_BOINC.cpp:819-844) and keeps
+// the best of the ten fits (:900-924).  Here, for every trial period:
+//   1. features: the phase-folded light curve at the period (ps_lc_features)
+//      and the asteroid's amplitude-aspect features (ps_aspect_features);
+//   2. the network maps them to a pole (doubled-longitude unit vector) and
+//      the l <= 2 shape coefficients; one start (and optionally its mirror
+//      pole, lambda + 180 deg with the m = 1 coefficients negated) is fitted
+//      by the full LM loop under the problem's stop rule;
+//   3. per block of trial periods, the n_candidates periods of lowest rms
+//      (no two within halo of each other) and halo periods on either side of
+//      each are refitted from the reference's ten poles by the judge engine
+//      (any precision; FP64 makes the judged rows the plain FP64 rows), while
+//      the next block's proposed fits run on the screening engine.
+// Judged rows replace the screened rows and are flagged; only they are
+// results in the reference's sense.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "periodsearch.h"
+#include "ps_internal.h"
+#include "ps_proposal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+int count_coefs(int lmax, int mmax) {
+  int n = 0;
+  for (int m = 0; m <= mmax; m++) n += (lmax - m + 1) * (m ? 2 : 1);
+  return n;
+}
+
+// Positions of (l, m, cos/sin) with l <= 2 in the coefficient vector (for
+// m = 0..mmax, l = m..lmax: one entry for m = 0, a cos/sin pair otherwise),
+// in the order c00, c10, c20, c11c, c11s, c21c, c21s, c22c, c22s; m1 marks
+// the m = 1 entries.  Needs lmax, mmax >= 2.
+void low_order_index(int lmax, int mmax, int idx[9], bool m1[9]) {
+  const int want[9][3] = {{0, 0, 0}, {1, 0, 0}, {2, 0, 0}, {1, 1, 0}, {1, 1, 1},
+                          {2, 1, 0}, {2, 1, 1}, {2, 2, 0}, {2, 2, 1}};
+  for (int w = 0; w < 9; w++) {
+    int k = 0;
+    for (int m = 0; m <= mmax; m++)
+      for (int l = m; l <= lmax; l++) {
+        const int parts = m ? 2 : 1;
+        for (int s = 0; s < parts; s++, k++)
+          if (l == want[w][0] && m == want[w][1] && s == want[w][2]) idx[w] = k;
+      }
+    m1[w] = want[w][1] == 1;
+  }
+}
+
+// The reference's pole reduction (period_search_BOINC.cpp:913-924) over
+// per fits per trial period: the first fit whose dev is strictly below the
+// best so far (from 1e40) wins, so a NaN never does; lambda wrapped to
+// [0, 360); chi2 = dev^2 (ndata - 3).
+void rows_from_fits(const ps_fit_out *f, int n_periods, int per, int n_points, ps_row *rows) {
+  for (int i = 0; i < n_periods; i++) {
+    double dev_best = 1e40, per_best = 0, dark_best = 0, la_best = 0, be_best = 0;
+    for (int m = 0; m < per; m++) {
+      const ps_fit_out &o = f[(size_t)i * per + m];
+      if (o.dev < dev_best) {
+        dev_best = o.dev;
+        per_best = o.prd;
+        dark_best = o.dark / o.totarea * 100;
+        la_best = o.la;
+        be_best = o.be;
+      }
+    }
+    if (la_best < 0) la_best += 360;
+    ps_row &r = rows[i];
+    r.period_h = 24 * per_best;
+    r.rms = dev_best;
+    r.chisq = dev_best * dev_best * n_points;
+    r.dark = dark_best;
+    r.lambda = la_best;
+    r.beta = be_best;
+  }
+}
+
+// Positions (0-based within the block) of the n_candidates lowest-rms rows,
+// no two within halo of each other, each widened by halo on either side.
+std::vector<int> pick_candidates(const ps_row *rows, int n, int n_candidates, int halo) {
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  // NaN rms sort last
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const double ra = rows[a].rms, rb = rows[b].rms;
+    if (std::isnan(ra)) return false;
+    if (std::isnan(rb)) return true;
+    return ra < rb;
+  });
+  std::vector<int> picked;
+  for (int k : order) {
+    if ((int)picked.size() >= n_candidates) break;
+    bool near = false;
+    for (int q : picked) near = near || std::abs(k - q) <= halo;
+    if (!near) picked.push_back(k);
+  }
+  std::vector<char> mark(n, 0);
+  for (int k : picked)
+    for (int j = std::max(0, k - halo); j <= std::min(n - 1, k + halo); j++) mark[j] = 1;
+  std::vector<int> out;
+  for (int i = 0; i < n; i++)
+    if (mark[i]) out.push_back(i);
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps_rows_from_fits(const ps_fit_out *fits, int n_periods, int per, int n_points, ps_row *rows) {
+  if (!fits || !rows || n_periods < 0 || per < 1 || n_points < 1) return PS_ERR_ARG;
+  rows_from_fits(fits, n_periods, per, n_points, rows);
+  return PS_OK;
+}
+
+int ps_aspect_features(const ps_problem *p, float out[5]) {
+  if (!p || !out || p->n_points < 1 || p->n_lc < 1 || !p->tim || !p->brightness || !p->ee || !p->lc_points)
+    return PS_ERR_ARG;
+  std::vector<double> amp, lon;
+  int q = 0;
+  for (int i = 0; i < p->n_lc; i++) {
+    const int k = p->lc_points[i];
+    const int q0 = q;
+    q += k;
+    if (k < 10) continue;
+    double tmin = p->tim[q0], tmax = tmin, ymin = p->brightness[q0], ymax = ymin, ysum = 0, e0 = 0, e1 = 0;
+    for (int j = q0; j < q; j++) {
+      tmin = std::min(tmin, p->tim[j]);
+      tmax = std::max(tmax, p->tim[j]);
+      ymin = std::min(ymin, p->brightness[j]);
+      ymax = std::max(ymax, p->brightness[j]);
+      ysum += p->brightness[j];
+      e0 += p->ee[3 * j];
+      e1 += p->ee[3 * j + 1];
+    }
+    if (tmax - tmin > 1.0) continue;
+    const double m = ysum / k;
+    if (!(m > 0)) continue;
+    amp.push_back((ymax - ymin) / m);
+    lon.push_back(std::atan2(e1 / k, e0 / k));
+  }
+  for (int i = 0; i < 5; i++) out[i] = 0.0f;
+  const size_t n = amp.size();
+  if (n >= 2) {
+    double am = 0;
+    for (double a : amp) am += a;
+    am /= n;
+    double c = 0, s = 0;
+    for (size_t i = 0; i < n; i++) {
+      c += (amp[i] - am) * std::cos(2 * lon[i]);
+      s += (amp[i] - am) * std::sin(2 * lon[i]);
+    }
+    c /= n * am;
+    s /= n * am;
+    out[0] = (float)c;
+    out[1] = (float)s;
+    out[2] = (float)std::hypot(c, s);
+    out[3] = (float)am;
+    out[4] = (float)(n / 10.0);
+  }
+  return PS_OK;
+}
+
+int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *net, const ps_learned_opts *opt,
+                    double freq_start, double freq_step, int n_from, int n_to, double conw_r, ps_row *rows,
+                    int *judged, ps_row *screened, long long *n_fits) {
+  if (!e || !p || !net || !net->mlp || !opt || !rows || !judged || n_from < 1 || n_to < n_from || opt->block < 1 ||
+      opt->n_candidates < 0 || opt->halo < 0 || net->nbins < 1 || net->nbins > 256 || net->max_batch < 1 || !net->x_mean ||
+      !net->x_std || !net->t_mean || !net->t_std)
+    return PS_ERR_ARG;
+  if (p->lmax < 2 || p->mmax < 2) {
+    psamd_engine_error(e, "the learned start needs lmax, mmax >= 2");
+    return PS_ERR_ARG;
+  }
+  int nc = 0, ma = 0, mfit = 0;
+  int rc = ps_dims(e, &nc, &ma, &mfit);
+  if (rc) return rc;
+  if (nc != count_coefs(p->lmax, p->mmax)) {
+    psamd_engine_error(e, "ps_scan_learned: the problem is not the engine's");
+    return PS_ERR_ARG;
+  }
+  const int n_in = net->nbins + 2 + 5, n_out = 12;
+  if (net->n_in != n_in || net->n_out != n_out) {
+    psamd_engine_error(e, "ps_scan_learned: the network's shape does not match the features");
+    return PS_ERR_ARG;
+  }
+  ps_engine *judge = opt->judge ? opt->judge : e;
+  const int per = opt->mirror ? 2 : 1;
+  const int npol = p->n_poles;
+  int idx[9];
+  bool m1[9];
+  low_order_index(p->lmax, p->mmax, idx, m1);
+  float aspect[5];
+  if ((rc = ps_aspect_features(p, aspect))) return rc;
+  const int N = n_to - n_from + 1;
+  std::vector<double> freq(N);
+  for (int i = 0; i < N; i++) freq[i] = freq_start - (n_from + i - 1) * freq_step;  // period_search_BOINC.cpp:812
+  long long fits_run = 0;
+  for (int i = 0; i < N; i++) judged[i] = 0;
+
+  // judge work of the previous block, run on its own thread when the judge
+  // is another engine (so its fits overlap the next block's screening fits)
+  struct Judge {
+    std::vector<int> pos;  // positions in [0, N)
+    std::vector<double> f;
+    std::vector<int> pole;
+    std::vector<ps_fit_out> out;
+    int rc = PS_OK;
+  } running;
+  std::thread worker;
+  auto start_judge = [&](Judge &J) {
+    J.f.clear();
+    J.pole.clear();
+    for (int i : J.pos)
+      for (int m = 0; m < npol; m++) {
+        J.f.push_back(freq[i]);
+        J.pole.push_back(m);
+      }
+    J.out.assign(J.f.size(), ps_fit_out{});
+    auto run = [&J, judge, conw_r]() {
+      J.rc = J.f.empty() ? PS_OK : ps_fit_batch(judge, J.f.data(), J.pole.data(), (int)J.f.size(), conw_r, J.out.data());
+    };
+    if (judge != e)
+      worker = std::thread(run);
+    else
+      run();
+  };
+  auto finish_judge = [&](Judge &J) -> int {
+    if (worker.joinable()) worker.join();
+    if (J.rc) return J.rc;
+    fits_run += (long long)J.out.size();
+    std::vector<ps_row> r(J.pos.size());
+    rows_from_fits(J.out.data(), (int)J.pos.size(), npol, p->n_points, r.data());
+    for (size_t k = 0; k < J.pos.size(); k++) {
+      rows[J.pos[k]] = r[k];
+      judged[J.pos[k]] = 1;
+    }
+    J.pos.clear();
+    return PS_OK;
+  };
+
+  std::vector<float> feat, x, y;
+  std::vector<double> cg;
+  std::vector<ps_fit_out> out;
+  std::vector<ps_row> brow;
+  const double *cgf = p->cg_first;
+  for (int b0 = 0; b0 < N; b0 += opt->block) {
+    const int b1 = std::min(N, b0 + opt->block), nb = b1 - b0;
+    // 1. features and the network
+    feat.assign((size_t)nb * (net->nbins + 2), 0.0f);
+    if ((rc = ps_lc_features(e, freq.data() + b0, nb, net->nbins, feat.data()))) break;
+    x.assign((size_t)nb * n_in, 0.0f);
+    for (int i = 0; i < nb; i++) {
+      float *xi = x.data() + (size_t)i * n_in;
+      for (int k = 0; k < net->nbins + 2; k++) xi[k] = feat[(size_t)i * (net->nbins + 2) + k];
+      for (int k = 0; k < 5; k++) xi[net->nbins + 2 + k] = aspect[k];
+      for (int k = 0; k < n_in; k++) xi[k] = (xi[k] - net->x_mean[k]) / net->x_std[k];
+    }
+    y.assign((size_t)nb * n_out, 0.0f);
+    for (int i0 = 0; i0 < nb; i0 += net->max_batch) {
+      const int nn = std::min(net->max_batch, nb - i0);
+      if ((rc = ps_mlp_forward(net->mlp, x.data() + (size_t)i0 * n_in, nn, y.data() + (size_t)i0 * n_out))) {
+        psamd_engine_error(e, std::string("network: ") + ps_mlp_last_error(net->mlp));
+        break;
+      }
+    }
+    if (rc) break;
+    // 2. starts: the reference's start (period_search_BOINC.cpp:819-844)
+    //    with the proposed pole and l <= 2 coefficients (c00, the size, kept)
+    cg.assign((size_t)nb * per * ma, 0.0);
+    for (int i = 0; i < nb; i++) {
+      double yv[12];
+      for (int k = 0; k < n_out; k++)
+        yv[k] = (double)(y[(size_t)i * n_out + k] * net->t_std[k] + net->t_mean[k]);
+      const double vn = std::max(std::sqrt(yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2]), 1e-12);
+      const double beta = std::asin(std::min(1.0, std::max(-1.0, yv[2] / vn)));
+      const double lam = 0.5 * std::atan2(yv[1], yv[0]);
+      for (int k = 0; k < per; k++) {
+        double *a = cg.data() + ((size_t)i * per + k) * ma;
+        for (int c = 0; c < nc; c++) a[c] = cgf[c];
+        for (int w = 1; w < 9; w++) a[idx[w]] = (k == 1 && m1[w]) ? -yv[3 + w] : yv[3 + w];
+        a[nc] = kPi / 2 - beta;
+        a[nc + 1] = std::fmod(lam + k * kPi, 2 * kPi);
+        if (a[nc + 1] < 0) a[nc + 1] += 2 * kPi;
+        a[nc + 2] = 24 * 2 * kPi / (1 / freq[b0 + i]);
+        a[nc + 3] = p->par[0];
+        a[nc + 4] = p->par[1];
+        a[nc + 5] = p->par[2];
+        a[nc + 6] = std::log(p->cl);
+        a[nc + 7] = 1;
+      }
+    }
+    // 3. screening fits (the previous block's judge runs meanwhile)
+    out.assign((size_t)nb * per, ps_fit_out{});
+    rc = ps_fit_batch_from(e, cg.data(), nb * per, conw_r, out.data(), nullptr);
+    const int rc_j = finish_judge(running);
+    if (rc) break;
+    if ((rc = rc_j)) break;
+    fits_run += (long long)nb * per;
+    brow.assign(nb, ps_row{});
+    rows_from_fits(out.data(), nb, per, p->n_points, brow.data());
+    for (int i = 0; i < nb; i++) {
+      rows[b0 + i] = brow[i];
+      if (screened) screened[b0 + i] = brow[i];
+    }
+    // 4. candidates of this block go to the judge
+    std::vector<int> cand = pick_candidates(brow.data(), nb, opt->n_candidates, opt->halo);
+    running.pos.clear();
+    for (int i : cand) running.pos.push_back(b0 + i);
+    start_judge(running);
+  }
+  if (rc) {
+    if (worker.joinable()) worker.join();
+    return rc;
+  }
+  if ((rc = finish_judge(running))) return rc;
+  if (n_fits) *n_fits = fits_run;
+  return PS_OK;
+}
+
+}  // extern "C"

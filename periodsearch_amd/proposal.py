@@ -32,6 +32,7 @@ scalings in an .npz next to this file (proposal_net.npz).
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import os
 import sys
 import tempfile
@@ -39,7 +40,7 @@ import time
 
 import numpy as np
 
-from . import FIT_DTYPE, ROW_DTYPE, Engine, Input, Mlp, n_coef, synth
+from . import FIT_DTYPE, ROW_DTYPE, Engine, Input, Mlp, PsLearnedOpts, PsProposalNet, lib, n_coef, synth
 
 NBINS = 64
 N_ASPECT = 5
@@ -72,7 +73,8 @@ def low_order_index(lmax: int, mmax: int):
 
 
 def aspect_features(inp: Input) -> np.ndarray:
-    """Amplitude-aspect features of one asteroid (the same at every trial period).
+    """Amplitude-aspect features of one asteroid (the same at every trial
+    period), computed by the library (ps_aspect_features).
 
     A light curve's amplitude grows with the aspect angle between the spin
     axis and the line of sight.  For views near the ecliptic it varies with
@@ -84,32 +86,10 @@ def aspect_features(inp: Input) -> np.ndarray:
     direction; features [c, s] = sum (A - mean A) (cos 2lv, sin 2lv) /
     (count * mean A), their length, the mean amplitude, and the count of dense
     light curves / 10.  Zeros with fewer than two dense light curves."""
-    p = inp.problem
-    n = p.n_points
-    t = np.ctypeslib.as_array(p.tim, shape=(n,))
-    y = np.ctypeslib.as_array(p.brightness, shape=(n,))
-    ee = np.ctypeslib.as_array(p.ee, shape=(3 * n,)).reshape(n, 3)
-    counts = np.ctypeslib.as_array(p.lc_points, shape=(p.n_lc,))
-    amp, lon = [], []
-    q = 0
-    for k in counts:
-        sl = slice(q, q + int(k))
-        q += int(k)
-        if k < 10 or np.ptp(t[sl]) > 1.0:
-            continue
-        m = y[sl].mean()
-        if not m > 0:
-            continue
-        amp.append(np.ptp(y[sl]) / m)
-        e = ee[sl].mean(0)
-        lon.append(np.arctan2(e[1], e[0]))
     out = np.zeros(N_ASPECT, np.float32)
-    if len(amp) >= 2:
-        a, lv = np.array(amp), np.array(lon)
-        am = a.mean()
-        c = float(np.sum((a - am) * np.cos(2 * lv)) / (len(a) * am))
-        s = float(np.sum((a - am) * np.sin(2 * lv)) / (len(a) * am))
-        out[:] = [c, s, np.hypot(c, s), am, len(a) / 10.0]
+    rc = lib.ps_aspect_features(C.byref(inp.problem), out.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        raise RuntimeError(f"ps_aspect_features failed: {rc}")
     return out
 
 
@@ -160,10 +140,14 @@ class Proposal:
     """A trained network with its input and target scalings."""
 
     def __init__(self, params: np.ndarray, x_mean, x_std, t_mean, t_std, max_batch: int = 4096, device: int = 0):
-        self.x_mean, self.x_std = np.asarray(x_mean, np.float32), np.asarray(x_std, np.float32)
-        self.t_mean, self.t_std = np.asarray(t_mean, np.float32), np.asarray(t_std, np.float32)
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        self.x_mean, self.x_std = f32(x_mean), f32(x_std)
+        self.t_mean, self.t_std = f32(t_mean), f32(t_std)
         self.net = Mlp(N_IN, HIDDEN, N_OUT, max_batch=max_batch, device=device)
         self.net.set_params(params)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self.c_net = PsProposalNet(self.net._h, N_IN, N_OUT, NBINS, max_batch, fp(self.x_mean), fp(self.x_std),
+                                   fp(self.t_mean), fp(self.t_std))
 
     @classmethod
     def load(cls, path: str = DEFAULT_NET, **kw) -> "Proposal":
@@ -205,94 +189,37 @@ class Proposal:
         self.net.close()
 
 
-def _rows_from_fits(fits: np.ndarray, per: int, n_points: int) -> np.ndarray:
-    """Best-of-`per` rows (the pole reduction of period_search_BOINC.cpp:900-924:
-    the first fit with the smallest rms, lambda wrapped to [0, 360),
-    chi2 = rms^2 (ndata - 3) with ndata = n_points + the 3 convexity rows)."""
-    f = fits.reshape(-1, per)
-    # the reference keeps the first fit whose dev is strictly below the best
-    # so far, starting from 1e40: a NaN dev (a diverged fit) never wins, and
-    # if no fit qualifies the row keeps dev 1e40 and zeros
-    # (period_search_BOINC.cpp:913-924)
-    dev = f["dev"]
-    ok = dev < 1e40  # False for NaN
-    key = np.where(ok, dev, np.inf)
-    k = np.argmin(key, axis=1)
-    any_ok = ok.any(axis=1)
-    b = f[np.arange(len(f)), k]
-    rows = np.zeros(len(f), dtype=ROW_DTYPE)
-    dev_b = np.where(any_ok, b["dev"], 1e40)
-    rows["period_h"] = np.where(any_ok, 24 * b["prd"], 0.0)
-    rows["rms"] = dev_b
-    rows["chisq"] = dev_b ** 2 * n_points
-    with np.errstate(invalid="ignore", divide="ignore"):
-        rows["dark"] = np.where(any_ok, b["dark"] / b["totarea"] * 100, 0.0)
-    la = np.where(any_ok, b["la"], 0.0)
-    rows["lambda"] = np.where(la < 0, la + 360, la)
-    rows["beta"] = np.where(any_ok, b["be"], 0.0)
-    return rows
-
-
-def _pick_candidates(rms: np.ndarray, n_candidates: int, halo: int) -> np.ndarray:
-    """Mask of the n_candidates lowest-rms rows (no two within halo) and
-    halo rows on either side of each."""
-    verified = np.zeros(len(rms), bool)
-    picked = []
-    for k in np.argsort(rms, kind="stable"):
-        if len(picked) >= n_candidates:
-            break
-        if any(abs(int(k) - q) <= halo for q in picked):
-            continue
-        picked.append(int(k))
-    for k in picked:
-        verified[max(0, k - halo):min(len(rms), k + halo + 1)] = True
-    return verified
+def _validator_ok(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Per row pair, the project validator's acceptance rule
+    (validator/period_search_validator4.cpp:88-103): |(x - y) / (x + y)| / 2
+    at most 0.1 for the period and the rms, 0.5 for chi2."""
+    def ok(x, y, tol):
+        return np.abs((x - y) / (x + y)) / 2 <= tol
+    return ok(a["period_h"], b["period_h"], 0.1) & ok(a["rms"], b["rms"], 0.1) & ok(a["chisq"], b["chisq"], 0.5)
 
 
 def scan_learned(eng: Engine, inp: Input, prop: Proposal, conw_r: float, n_from: int, n_to: int,
-                 n_candidates: int = 4, halo: int = 2, mirror: bool = False, block: int = 1024):
-    """Rows n_from..n_to, in blocks of `block` trial periods: every period
-    fitted from the proposed start (and, if asked, its mirror pole); in each
-    block the n_candidates periods with the lowest rms and halo periods on
-    either side of each are refitted from the reference's ten poles.  Those
-    refits run in the same batch as the next block's proposed fits (one
-    batch of LM fits per block instead of two).  Returns (rows, verified
-    mask, number of LM fits run, rows of the proposed starts alone); only
-    verified rows are judged results."""
-    n = np.arange(n_from, n_to + 1)
-    freq = inp.freq_start - (n - 1) * inp.freq_step
-    per = 2 if mirror else 1
-    npol = inp.problem.n_poles
-    learned = np.zeros(len(n), dtype=ROW_DTYPE)
-    rows = np.zeros(len(n), dtype=ROW_DTYPE)
-    verified = np.zeros(len(n), bool)
-    n_fits = 0
-    pending = np.zeros(0, int)  # positions of the previous block awaiting their plain refits
-    blocks = list(range(0, len(n), block))
-    for bi in range(len(blocks) + 1):
-        parts = []
-        if bi < len(blocks):
-            b0, b1 = blocks[bi], min(len(n), blocks[bi] + block)
-            starts = prop.starts(inp, freq[b0:b1], features(eng, inp, freq[b0:b1]))
-            parts.append(starts if mirror else starts[0::2])
-        if len(pending):
-            parts.append(reference_starts(inp, freq[pending]))
-        if not parts:
-            break
-        fits = eng.fit_batch_from(np.concatenate(parts), conw_r)
-        n_fits += len(fits)
-        m = 0
-        new_pending = np.zeros(0, int)
-        if bi < len(blocks):
-            m = (b1 - b0) * per
-            learned[b0:b1] = _rows_from_fits(fits[:m], per, inp.ndata)
-            rows[b0:b1] = learned[b0:b1]
-            new_pending = b0 + np.flatnonzero(_pick_candidates(learned["rms"][b0:b1], n_candidates, halo))
-        if len(pending):
-            rows[pending] = _rows_from_fits(fits[m:], npol, inp.ndata)
-            verified[pending] = True
-        pending = new_pending
-    return rows, verified, n_fits, learned
+                 n_candidates: int = 4, halo: int = 2, mirror: bool = False, block: int = 1024,
+                 judge: Engine | None = None):
+    """Rows n_from..n_to of the learned scan (ps_scan_learned): every trial
+    period fitted from the proposed start (and, if asked, its mirror pole);
+    per block of `block` periods the n_candidates periods of lowest rms and
+    halo periods on either side of each are refitted from the reference's
+    ten poles by `judge` (another engine with the same problem, e.g. in FP64,
+    working while the next block is screened) or by eng.  Returns (rows,
+    judged mask, number of LM fits run, rows of the proposed starts alone);
+    only judged rows are results in the reference's sense."""
+    n = n_to - n_from + 1
+    rows = np.zeros(n, dtype=ROW_DTYPE)
+    screened = np.zeros(n, dtype=ROW_DTYPE)
+    judged = np.zeros(n, dtype=np.int32)
+    nf = C.c_longlong()
+    opts = PsLearnedOpts(block, n_candidates, halo, 1 if mirror else 0, judge._h if judge is not None else None)
+    eng._check(lib.ps_scan_learned(eng._h, C.byref(inp.problem), C.byref(prop.c_net), C.byref(opts),
+                                   inp.freq_start, inp.freq_step, n_from, n_to, conw_r, rows.ctypes.data,
+                                   judged.ctypes.data_as(C.POINTER(C.c_int)), screened.ctypes.data, C.byref(nf)),
+               "ps_scan_learned")
+    return rows, judged.astype(bool), nf.value, screened
 
 
 # ---------------------------------------------------------------- training
@@ -381,16 +308,23 @@ def train(x: np.ndarray, t: np.ndarray, epochs: int = 200, batch: int = 512, lr:
 
 
 def evaluate(prop: Proposal, n_asteroids: int = 16, seed0: int = 5000, window: int = 200, cfg_over=None,
-             mirror: bool = False, n_candidates: int = 4, stop: float = 1e-4, log=print) -> dict:
+             mirror: bool = False, n_candidates: int = 4, halo: int = 2, stop: float = 1e-4, prec: int = 0,
+             log=print) -> dict:
     """Held-out asteroids (seeds seed0..), each on a window of trial periods
     around its true period:
-      * the plain scan (ten poles per period) and scan_learned; whether the
-        best period (plain rows only, from both scans) agrees within one scan
-        step, the chi2 of the two minima, the LM fits each scan ran;
+      * the plain FP64 scan (ten poles per period) and scan_learned with
+        screening fits in precision `prec` and the judge fits in FP64 (for
+        prec = FP64 one engine does both); whether the best judged row is the
+        plain scan's best row (same trial period, and its chi2), the LM fits
+        each scan ran, and how every screened row compares with the plain row
+        of its period;
       * at the true period under a convergence stop rule (stop < 1): mean LM
         iterations per fit and the best chi2 from the ten plain starts and
         from the proposed starts."""
-    eng = Engine(0)
+    e64 = Engine(0)
+    eng = Engine(0) if prec else e64
+    if prec:
+        eng.set_precision(prec)
     res = []
     with tempfile.TemporaryDirectory() as d:
         for i in range(n_asteroids):
@@ -401,31 +335,36 @@ def evaluate(prop: Proposal, n_asteroids: int = 16, seed0: int = 5000, window: i
             path = os.path.join(d, "period_search_in")
             synth.write_input(path, ast, P * 0.97, P * 1.03)
             inp = Input(path)
-            eng.set_problem(inp.problem, keep=inp)
+            e64.set_problem(inp.problem, keep=inp)
+            if prec:
+                eng.set_problem(inp.problem, keep=inp)
             n_true = int(round((inp.freq_start - 1.0 / P) / inp.freq_step)) + 1
             n0, n1 = max(1, n_true - window // 2), min(inp.n_periods, n_true + window // 2)
             t0 = time.perf_counter()
-            plain = eng.scan(inp.freq_start, inp.freq_step, n0, n1, inp.conw_r0)
+            plain = e64.scan(inp.freq_start, inp.freq_step, n0, n1, inp.conw_r0)
             t_plain = time.perf_counter() - t0
             t0 = time.perf_counter()
             rows, ver, nf, learned = scan_learned(eng, inp, prop, inp.conw_r0, n0, n1, n_candidates=n_candidates,
-                                                  mirror=mirror)
+                                                  halo=halo, mirror=mirror, judge=e64 if prec else None)
             t_learn = time.perf_counter() - t0
-            kp = int(np.argmin(plain["chisq"]))
-            kl = int(np.flatnonzero(ver)[np.argmin(rows["chisq"][ver])])
+            kp = int(np.nanargmin(plain["chisq"]))
+            kl = int(np.flatnonzero(ver)[np.nanargmin(rows["chisq"][ver])])
+            rel = np.abs(learned["chisq"] - plain["chisq"]) / plain["chisq"]
             r = dict(seed=seed0 + i, n_periods=n1 - n0 + 1, best_plain=kp, best_learned=kl,
                      p_plain=float(plain["period_h"][kp]), p_learned=float(rows["period_h"][kl]),
                      chi2_plain=float(plain["chisq"][kp]), chi2_learned=float(rows["chisq"][kl]),
                      step_h=float(P * P * inp.freq_step), fits_plain=10 * (n1 - n0 + 1), fits_learned=nf,
                      t_plain=t_plain, t_learned=t_learn,
-                     screen_rank=int(np.sum(learned["rms"] < learned["rms"][kp])))
+                     screen_rank=int(np.sum(learned["rms"] < learned["rms"][kp])),
+                     screened_within_1e4=float(np.mean(rel <= 1e-4)),
+                     screened_within_validator=float(np.mean(_validator_ok(learned, plain))))
             # iterations to convergence at the true period
             synth.write_input(path, ast, P * 0.97, P * 1.03, stop=stop)
             inp2 = Input(path)
-            eng.set_problem(inp2.problem, keep=inp2)
+            e64.set_problem(inp2.problem, keep=inp2)
             f = np.array([inp2.freq_start - (n_true - 1) * inp2.freq_step])
-            fp = eng.fit_batch_from(reference_starts(inp2, f), inp2.conw_r0)
-            fl = eng.fit_batch_from(prop.starts(inp2, f, features(eng, inp2, f)), inp2.conw_r0)
+            fp = e64.fit_batch_from(reference_starts(inp2, f), inp2.conw_r0)
+            fl = e64.fit_batch_from(prop.starts(inp2, f, features(e64, inp2, f)), inp2.conw_r0)
             r.update(iter_plain=float(fp["niter"].mean()), iter_learned=float(fl["niter"].mean()),
                      conv_chi2_plain=float(fp["chisq"].min()), conv_chi2_learned=float(fl["chisq"].min()))
             res.append(r)
@@ -433,9 +372,14 @@ def evaluate(prop: Proposal, n_asteroids: int = 16, seed0: int = 5000, window: i
                 f"chi2 {r['chi2_plain']:.6g} / {r['chi2_learned']:.6g}, fits {r['fits_plain']} / {nf}; "
                 f"converged: iterations {r['iter_plain']:.1f} / {r['iter_learned']:.1f}, "
                 f"chi2 {r['conv_chi2_plain']:.6g} / {r['conv_chi2_learned']:.6g}")
-    eng.close()
+    if prec:
+        eng.close()
+    e64.close()
     return dict(runs=res,
                 same_period=float(np.mean([abs(r["p_plain"] - r["p_learned"]) <= r["step_h"] for r in res])),
+                same_row=float(np.mean([r["best_plain"] == r["best_learned"] for r in res])),
+                screened_within_1e4=float(np.mean([r["screened_within_1e4"] for r in res])),
+                screened_within_validator=float(np.mean([r["screened_within_validator"] for r in res])),
                 chi2_rel_max=float(max(abs(r["chi2_learned"] - r["chi2_plain"]) / r["chi2_plain"] for r in res)),
                 fit_ratio=float(sum(r["fits_plain"] for r in res) / sum(r["fits_learned"] for r in res)),
                 time_ratio=float(sum(r["t_plain"] for r in res) / sum(r["t_learned"] for r in res)),

@@ -139,13 +139,14 @@ def make_asteroid(seed: int, *, lmax: int, mmax: int, nrows: int, n_dense: int, 
 
 
 def write_input(path: str, ast: dict, per_start: float, per_end: float, step_coef: float = 0.5,
-                stop: float = 50, n_iter_min: int = 10) -> None:
-    """period_search_in with the reference's default header (period_search/period_search_in:1-14)."""
+                stop: float = 50, n_iter_min: int = 10, conw: float = 0.1) -> None:
+    """period_search_in with the reference's default header (period_search/period_search_in:1-14);
+    conw is the initial convexity weight (0.1 there)."""
     with open(path, "w") as f:
         f.write(f"{per_start!r} {step_coef!r} {per_end!r} 1\tperiod_start period_step period_end fixed/free\n")
         f.write("0.0\t\t\t\tinitial epoch\n")
         f.write("0.0\t\t\t\tinitial rotation angle\n")
-        f.write("0.1\t\t\t\tconvexity weight\n")
+        f.write(f"{conw!r}\t\t\t\tconvexity weight\n")
         f.write(f"{ast['lmax']} {ast['mmax']}\t\t\t\tdegree and order of spherical harmonics\n")
         f.write(f"{ast['nrows']}\t\t\t\tnumber of rows\n")
         f.write("0.5\t1\t\t\tscattering parameters\n0.1\t1\n-0.5\t1\n")
@@ -172,6 +173,11 @@ WORKLOADS = {
                span_years=15, period_h=7.3, per_start=2.0, per_end=100.0),
     "big": dict(seed=2, lmax=8, mmax=8, nrows=10, n_dense=20, dense_pts=150, n_sparse=2000, noise=0.02,
                 span_years=8, period_h=7.3, per_start=4.0, per_end=12.0),
+    # the same asteroid observed over two apparitions 420 days apart, so that
+    # the reference's step rule gives 6,720 trial periods over 4-12 h (one
+    # apparition, 190 days, would give 3,037)
+    "c2": dict(seed=2, lmax=8, mmax=8, nrows=10, n_dense=20, dense_pts=150, n_sparse=2000, noise=0.02,
+               span_years=1.15, period_h=7.3, per_start=4.0, per_end=12.0),
 }
 
 

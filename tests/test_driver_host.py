@@ -11,7 +11,6 @@ import pytest
 
 import oracle
 import periodsearch_amd as ps
-from periodsearch_amd import proposal
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_WU = "/root/reference/period_search/input_period_search"
@@ -100,11 +99,12 @@ def _fits(devs):
     return f
 
 
-def test_learned_rows_pole_reduction_skips_nan():
-    """period_search_BOINC.cpp:913-924: the first strictly smaller dev wins,
-    starting from 1e40; NaN never wins; with none, dev 1e40 and period 0."""
+def test_pole_reduction_skips_nan():
+    """period_search_BOINC.cpp:913-924 (ps_rows_from_fits, used by the learned
+    scan): the first strictly smaller dev wins, starting from 1e40; NaN never
+    wins; with none, dev 1e40 and period 0."""
     nan = float("nan")
-    rows = proposal._rows_from_fits(np.concatenate([_fits([nan, 0.5, 0.2, 0.2]), _fits([nan] * 4),
+    rows = ps.rows_from_fits(np.concatenate([_fits([nan, 0.5, 0.2, 0.2]), _fits([nan] * 4),
                                                     _fits([0.3, nan, 0.1, nan])]), 4, 100)
     assert rows["rms"][0] == 0.2 and np.isclose(rows["period_h"][0], 24 * 0.32)
     assert rows["rms"][1] == 1e40 and rows["period_h"][1] == 0.0

@@ -3,10 +3,10 @@
 Tolerances:
   * one model evaluation (ps_eval_mrqcof), FP64: alpha, beta, chi2 and chck
     within 1e-12 relative (summation order and device libm differ from glibc);
-  * whole LM fits (ps_fit_batch), FP64: final chi2 within 1e-9 relative for at
-    least 80% of fits, median below 1e-12, none above 1e-5 -- 50 LM iterations
-    amplify last-bit differences on a few trajectories -- and the fitted
-    period within 1e-7;
+  * whole LM fits (ps_fit_batch), FP64: held per fit in test_gpu_loop.py,
+    against the oracle's own one-ulp spread (50 LM iterations amplify
+    last-bit differences on some trajectories, in the reference arithmetic
+    as much as here);
   * output rows, FP64: period 1e-9, rms and chi2 1e-6, pole within 0.01 deg;
   * mixed forms (FP32 facet stage, contraction and alpha on the matrix cores
     with FP32 accumulation; FP64 residuals, chi2, beta and solve): one
@@ -96,30 +96,6 @@ def test_eval_mrqcof_matches_oracle(engine, small_input):
         nor = orc.darr("Nor", (1001, 4))[1:F + 1, 1:4]
         ref_chck = area @ nor
         assert np.abs(chck[i] - ref_chck).max() < 1e-12 * area.sum(), (i, chck[i], ref_chck)
-
-
-def test_fit_batch_matches_oracle(engine, small_incl_input):
-    inp, orc = _setup(small_incl_input, engine)
-    freqs, poles = [], []
-    for n in (1, 40):
-        f = inp.freq_start - (n - 1) * inp.freq_step
-        for m in range(10):
-            freqs.append(f)
-            poles.append(m)
-    out = engine.fit_batch(freqs, poles, inp.conw_r0)
-    ref = [orc.fit(f, m + 1) for f, m in zip(freqs, poles)]
-    rel_chi = np.array([_rel(out["chisq"][k], r.chisq) for k, r in enumerate(ref)])
-    rel_prd = np.array([_rel(out["prd"][k], r.prd) for k, r in enumerate(ref)])
-    d_la = np.array([abs(out["la"][k] - r.la) for k, r in enumerate(ref)])
-    rel_area = np.array([_rel(out["totarea"][k], r.totarea) for k, r in enumerate(ref)])
-    print("fit chi2 rel: median %.2e p80 %.2e max %.2e; period rel max %.2e; |dlambda| max %.2e deg" % (
-        np.median(rel_chi), np.percentile(rel_chi, 80), rel_chi.max(), rel_prd.max(), d_la.max()))
-    assert all(out["niter"][k] == r.niter == 50 for k, r in enumerate(ref))
-    assert np.median(rel_chi) < 1e-12
-    assert (rel_chi < 1e-9).mean() >= 0.8
-    assert rel_chi.max() < 1e-5
-    assert rel_prd.max() < 1e-7
-    assert np.median(d_la) < 1e-6 and np.median(rel_area) < 1e-9
 
 
 def test_scan_rows_match_oracle(engine, small_incl_input):
