@@ -29,8 +29,9 @@ default path (poll graphs):
 With ``--form plain`` the value is the mixed form's own rate.
 
 After the timed regions: one profiled pass of the headline form (the model
-kernel timed per launch with HIP events captured into the poll graph) for
-the roofline figures; the matched-chi2 check (the headline's best judged row
+kernel timed per launch with HIP events, the LM iterations as plain launches,
+because events captured into a HIP graph cannot be timed here) for the
+roofline figures; the matched-chi2 check (the headline's best judged row
 against the plain FP64 scan's best row over the trial periods around the
 true period); and the reference's own fit loop (period_search/*.c compiled
 unchanged into oracle/_ref) timed on this host's cores on a bounded sample.
@@ -427,7 +428,9 @@ def main() -> int:
             "traffic": round(tb * evals_launch) if tb else None,
             "kernel": "model_mixed_kernel" if args.prec != "fp64" else "model_mfma_kernel",
             "ms_per_launch": round(ms_launch, 4), "evals_per_launch": round(evals_launch, 1),
-            "flop_per_eval": flop_eval, "timing": st["timing"],
+            "flop_per_eval": flop_eval,
+            "timing": "HIP events around each model launch, in a profiled pass after the timed regions (plain "
+                      "launches: events inside a HIP graph cannot be timed)",
             "vector": {"stage": "facet stage, %d flop per point-facet pair" % FACET_FLOP_PER_PAIR,
                        "achieved": round(vec, 3), "peak": vpeak, "unit": "TFLOP/s", "frac": round(vec / vpeak, 4),
                        "peak_without_packed_fp32": FP32_VECTOR_UNPACKED_TFLOPS if args.prec != "fp64" else None},

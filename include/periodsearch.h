@@ -208,15 +208,12 @@ int ps_last_batch(const ps_engine *e, double *ms, long long *items, long long *i
 typedef void (*ps_progress_fn)(void *user, long long fits_done, long long fits_total);
 int ps_set_progress(ps_engine *e, ps_progress_fn fn, void *user);
 /* Per-launch timing of the model kernel (the dominant kernel) with HIP
- * events on the engine's stream, and a count of model evaluations.
+ * events on the engine's stream, and a count of model evaluations.  While
+ * profiling, the LM iterations run as plain launches with events between
+ * them instead of replaying the poll graph (the kernels are the same).
  * ps_set_profiling resets the counters. */
 int ps_set_profiling(ps_engine *e, int on);
 int ps_kernel_stats(const ps_engine *e, double *model_ms, long long *model_launches, long long *evals);
-/* How the model kernel was timed since ps_set_profiling: 1 = events captured
- * as nodes of the poll graph (the default path, as run without profiling),
- * 2 = plain launches with events between them (PSAMD_GRAPH=0, or a runtime
- * that cannot time graph events), 0 = nothing timed yet. */
-int ps_kernel_timing_mode(const ps_engine *e);
 
 /* ---- host-side driver helpers ---------------------------------------- */
 /* Parsed and normalised period_search_in (period_search_BOINC.cpp:277-485). */

@@ -9,7 +9,9 @@
  *     same loop as ps_fit_batch, which takes the reference's starts);
  *   - ps_mlp_*: a small fully connected FP32 network (n_in -> H -> H -> n_out,
  *     ReLU) with GPU forward, backward and Adam kernels, which maps the
- *     features to a starting pole and low-order shape.
+ *     features to a starting pole and low-order shape;
+ *   - ps_scan_learned: the scan that screens every trial period from the
+ *     network's start and has the plain ten-pole fits judge the best.
  * Return codes are the PS_* codes of periodsearch.h. */
 #ifndef PS_PROPOSAL_H
 #define PS_PROPOSAL_H
@@ -84,6 +86,8 @@ typedef struct ps_learned_opts {
   int n_candidates;  /* lowest-rms periods per block sent to the judge (no two within halo) */
   int halo;          /* neighbours on either side of each candidate also judged */
   int mirror;        /* 1: fit the mirror pole (lambda + 180 deg) as well */
+  int refine;        /* rounds that judge the unjudged neighbours of the best judged row, until it is a local
+                      * minimum of the judged rows (0: none) */
   ps_engine *judge;  /* engine for the plain ten-pole judge fits (its precision, e.g. FP64, and the same
                       * problem); NULL: the screening engine.  Another engine runs a block's judge fits on
                       * its own thread while the next block is screened. */
@@ -96,8 +100,10 @@ typedef struct ps_learned_opts {
  * pole and l <= 2 shape); per block the candidates and their neighbours are
  * refitted from the reference's ten poles by the judge, and those rows
  * replace the screened ones: judged[i] = 1 marks them, the rows in the
- * reference's sense.  The other rows are the screening fits' (period of the
- * best screening fit, its rms and chi2).  p is the problem given to
+ * reference's sense; then the best judged row's unjudged neighbours are
+ * judged until it is a local minimum of the judged rows (opt->refine rounds
+ * at most).  The other rows are the screening fits' (period of the best
+ * screening fit, its rms and chi2).  p is the problem given to
  * ps_set_problem of e (and of the judge).  screened (optional, [n]) receives
  * every period's screening row; *n_fits the LM fits run (screening + judge). */
 int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *net, const ps_learned_opts *opt,

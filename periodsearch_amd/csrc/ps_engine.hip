@@ -2167,8 +2167,6 @@ struct ps_engine {
   // optional per-launch timing of the model kernel (HIP events on the
   // engine's stream)
   bool profile = false;
-  bool graph_events_ok = true;  // timing events can be captured into the poll graph
-  int timing_mode = 0;          // how model_ms was measured: 1 graph event nodes, 2 plain launches
   // progress callback (ps_set_progress), called from the host poll loop
   ps_progress_fn progress = nullptr;
   void *progress_user = nullptr;
@@ -2322,11 +2320,23 @@ int ensure_slots(ps_engine *e, int nslots) {
 }
 
 // Run W.n_items items through the slots until all are done.
-int run_batch(ps_engine *e, DevWork W) {
+// Slot state, slot memory and counters taken from caller-provided device
+// memory instead of the engine's own allocations.
+struct FixedSlots {
+  SlotState *st;
+  double *mem;
+  int *counters;
+  int max_slots;
+  long long slot_doubles;
+};
+
+int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
   if (W.n_items <= 0) return PS_OK;
   // as few refill rounds as max_slots allows, with the items spread evenly
   // over them (fits of equal length then finish together, no thin tail)
-  const int nrounds = (W.n_items + e->max_slots - 1) / e->max_slots;
+  const int max_slots = fixed ? fixed->max_slots : e->max_slots;
+  const int nrounds = (W.n_items + max_slots - 1) / max_slots;
+  int *const counters = fixed ? fixed->counters : e->counters;
   const DevProblem &P0 = e->P;  // the problem when there is one
   const DevProblem *probs = e->d_probs;
   // Lanes: each drives its own slots with its own kernel set on its own
@@ -2368,8 +2378,8 @@ int run_batch(ps_engine *e, DevWork W) {
       L.W.item_prob = W.item_prob + start;
       L.W.out = W.out + start;
       L.W.nslots = (n_c + nrounds - 1) / nrounds;
-      L.W.queue = e->counters + 3 * nl;  // per lane: queue, fits done, evaluations
-      L.W.done = e->counters + 3 * nl + 1;
+      L.W.queue = counters + 3 * nl;  // per lane: queue, fits done, evaluations
+      L.W.done = counters + 3 * nl + 1;
       nslots += L.W.nslots;
       nl++;
     }
@@ -2381,17 +2391,20 @@ int run_batch(ps_engine *e, DevWork W) {
       lane[g].ks = &eks;
       lane[g].W = W;
       lane[g].W.nslots = nslots / ng + (g < nslots % ng ? 1 : 0);
-      lane[g].W.queue = e->counters;
-      lane[g].W.done = e->counters + 1;
+      lane[g].W.queue = counters;
+      lane[g].W.done = counters + 1;
     }
     nl = ng;
   }
   int rc;
-  if ((rc = ensure_slots(e, nslots))) return rc;
+  if (!fixed && (rc = ensure_slots(e, nslots))) return rc;
+  SlotState *const st0 = fixed ? fixed->st : e->st;
+  double *const mem0 = fixed ? fixed->mem : e->mem;
+  const long long sd = fixed ? fixed->slot_doubles : e->slot_doubles;
   for (int l = 0, base = 0; l < nl; l++) {
-    lane[l].W.st = e->st + base;
-    lane[l].W.mem = e->mem + (size_t)base * e->slot_doubles;
-    lane[l].W.slot_doubles = e->slot_doubles;
+    lane[l].W.st = st0 + base;
+    lane[l].W.mem = mem0 + (size_t)base * sd;
+    lane[l].W.slot_doubles = sd;
     base += lane[l].W.nslots;
   }
   // clear every lane's slots and counters (shared counters: the same values again)
@@ -2407,7 +2420,9 @@ int run_batch(ps_engine *e, DevWork W) {
   const long long rounds = nrounds + 1;
   const long long guard = rounds * per_fit * 2 + 64;
   const int poll = 8;
-  if (e->profile && e->pev.size() < (size_t)(2 * kMaxGroups * poll)) {
+  // the fixed-memory path allocates nothing: no graph, no timing events
+  const bool profile = e->profile && !fixed;
+  if (profile && e->pev.size() < (size_t)(2 * kMaxGroups * poll)) {
     for (auto ev : e->pev) (void)hipEventDestroy(ev);
     e->pev.assign(2 * kMaxGroups * poll, nullptr);
     for (auto &ev : e->pev) HIP_TRY(hipEventCreate(&ev));
@@ -2440,11 +2455,11 @@ int run_batch(ps_engine *e, DevWork W) {
     return PS_OK;
   };
   hipGraphExec_t gexec = nullptr;
-  // With profiling the timing events are captured into the graph as event
-  // record nodes, so the timed path is the graph path; if the runtime cannot
-  // capture or time them, profiling falls back to plain launches with events
-  // between them (e->graph_events_ok false from then on).
-  if (e->use_graph && (!e->profile || e->graph_events_ok)) {
+  // Profiling times each model launch with HIP events between plain launches:
+  // this runtime cannot time events captured into a graph
+  // (hipEventElapsedTime on them returns hipErrorInvalidHandle), and the
+  // kernels' durations do not depend on how they are launched.
+  if (e->use_graph && !fixed && !profile) {
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(sg[0], hipStreamCaptureModeThreadLocal));
     // every call made while capturing is checked, but the capture is always
@@ -2453,24 +2468,18 @@ int run_batch(ps_engine *e, DevWork W) {
     auto capture_body = [&]() -> int {
       HIP_TRY(hipEventRecord(e->evf, sg[0]));  // fork the other lanes' streams into the capture
       for (int l = 1; l < nl; l++) HIP_TRY(hipStreamWaitEvent(sg[l], e->evf, 0));
-      return enqueue_poll(e->profile);
+      return enqueue_poll(false);
     };
     const int rc_cap = capture_body();
     hipError_t err_end = hipStreamEndCapture(sg[0], &graph);
-    hipError_t err_inst = hipSuccess;
-    if (!rc_cap && err_end == hipSuccess) err_inst = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (rc_cap || err_end != hipSuccess || err_inst != hipSuccess) {
-      gexec = nullptr;
-      if (!e->profile) {
-        if (rc_cap) return rc_cap;
-        HIP_TRY(err_end);
-        HIP_TRY(err_inst);
-      }
-      (void)hipGetLastError();
-      e->graph_events_ok = false;
-      e->err.clear();
+    if (rc_cap) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc_cap;
     }
+    HIP_TRY(err_end);
+    hipError_t err_inst = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(err_inst);
   }
   struct GraphGuard {
     hipGraphExec_t &g;
@@ -2485,33 +2494,20 @@ int run_batch(ps_engine *e, DevWork W) {
     if (gexec) {
       HIP_TRY(hipGraphLaunch(gexec, sg[0]));
     } else {
-      const int rc_p = enqueue_poll(e->profile);
+      const int rc_p = enqueue_poll(profile);
       if (rc_p) return rc_p;
     }
     it += poll;
-    HIP_TRY(hipMemcpyAsync(e->host_done, e->counters, ncnt * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->host_done, counters, ncnt * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
-    if (e->profile) {
-      double sum = 0;
-      bool ok = true;
-      for (int k = 0; k < poll && ok; k++)
-        for (int l = 0; l < nl && ok; l++) {
+    if (profile) {
+      for (int k = 0; k < poll; k++)
+        for (int l = 0; l < nl; l++) {
           float ms = 0;
-          ok = hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + l)], e->pev[2 * (kMaxGroups * k + l) + 1]) ==
-               hipSuccess;
-          sum += ms;
+          HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * (kMaxGroups * k + l)], e->pev[2 * (kMaxGroups * k + l) + 1]));
+          e->model_ms += ms;
         }
-      if (ok) {
-        e->model_ms += sum;
-        e->model_launches += poll * nl;
-        e->timing_mode = gexec ? 1 : 2;
-      } else if (gexec) {
-        (void)hipGetLastError();  // event nodes not timed by this runtime: plain launches from the next batch
-        e->graph_events_ok = false;
-      } else {
-        e->err = "timing the model kernel with HIP events failed";
-        return PS_ERR_HIP;
-      }
+      e->model_launches += poll * nl;
     }
     // host_done: queue, fits done, evaluations (per lane with size classes)
     bool all = true;
@@ -2991,14 +2987,11 @@ int ps_set_progress(ps_engine *e, ps_progress_fn fn, void *user) {
 int ps_set_profiling(ps_engine *e, int on) {
   if (!e) return PS_ERR_ARG;
   e->profile = on != 0;
-  e->timing_mode = 0;
   e->model_ms = 0;
   e->model_launches = 0;
   e->evals = 0;
   return PS_OK;
 }
-
-int ps_kernel_timing_mode(const ps_engine *e) { return e ? e->timing_mode : PS_ERR_ARG; }
 
 int ps_kernel_stats(const ps_engine *e, double *model_ms, long long *model_launches, long long *evals) {
   if (!e) return PS_ERR_ARG;

@@ -17,7 +17,9 @@
 //      (no two within halo of each other) and halo periods on either side of
 //      each are refitted from the reference's ten poles by the judge engine
 //      (any precision; FP64 makes the judged rows the plain FP64 rows), while
-//      the next block's proposed fits run on the screening engine.
+//      the next block's proposed fits run on the screening engine;
+//   4. the best judged row is walked to a local minimum of the judged chi2
+//      profile (its unjudged neighbours judged, a few at a time).
 // Judged rows replace the screened rows and are flagged; only they are
 // results in the reference's sense.
 #include <algorithm>
@@ -178,7 +180,7 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
                     double freq_start, double freq_step, int n_from, int n_to, double conw_r, ps_row *rows,
                     int *judged, ps_row *screened, long long *n_fits) {
   if (!e || !p || !net || !net->mlp || !opt || !rows || !judged || n_from < 1 || n_to < n_from || opt->block < 1 ||
-      opt->n_candidates < 0 || opt->halo < 0 || net->nbins < 1 || net->nbins > 256 || net->max_batch < 1 || !net->x_mean ||
+      opt->n_candidates < 0 || opt->halo < 0 || opt->refine < 0 || net->nbins < 1 || net->nbins > 256 || net->max_batch < 1 || !net->x_mean ||
       !net->x_std || !net->t_mean || !net->t_std)
     return PS_ERR_ARG;
   if (p->lmax < 2 || p->mmax < 2) {
@@ -327,6 +329,23 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
     return rc;
   }
   if ((rc = finish_judge(running))) return rc;
+  // 5. the judged rows' chi2 over neighbouring trial periods is smooth near a
+  //    minimum (neighbours converge to the same solution), so the best judged
+  //    row is walked to a local minimum of the kept profile: while a
+  //    neighbour of the best judged row is unjudged, judge it (at most
+  //    opt->refine rounds)
+  for (int round = 0; round < opt->refine; round++) {
+    int best = -1;
+    for (int i = 0; i < N; i++)
+      if (judged[i] && !std::isnan(rows[i].chisq) && (best < 0 || rows[i].chisq < rows[best].chisq)) best = i;
+    if (best < 0) break;
+    running.pos.clear();
+    for (int j : {best - 1, best + 1})
+      if (j >= 0 && j < N && !judged[j]) running.pos.push_back(j);
+    if (running.pos.empty()) break;
+    start_judge(running);
+    if ((rc = finish_judge(running))) return rc;
+  }
   if (n_fits) *n_fits = fits_run;
   return PS_OK;
 }

@@ -200,13 +200,15 @@ def _validator_ok(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 def scan_learned(eng: Engine, inp: Input, prop: Proposal, conw_r: float, n_from: int, n_to: int,
                  n_candidates: int = 4, halo: int = 2, mirror: bool = False, block: int = 1024,
-                 judge: Engine | None = None):
+                 judge: Engine | None = None, refine: int = 8):
     """Rows n_from..n_to of the learned scan (ps_scan_learned): every trial
     period fitted from the proposed start (and, if asked, its mirror pole);
     per block of `block` periods the n_candidates periods of lowest rms and
     halo periods on either side of each are refitted from the reference's
     ten poles by `judge` (another engine with the same problem, e.g. in FP64,
-    working while the next block is screened) or by eng.  Returns (rows,
+    working while the next block is screened) or by eng; then the best
+    judged row's unjudged neighbours are judged until it is a local minimum
+    of the judged rows (at most `refine` rounds).  Returns (rows,
     judged mask, number of LM fits run, rows of the proposed starts alone);
     only judged rows are results in the reference's sense."""
     n = n_to - n_from + 1
@@ -214,7 +216,8 @@ def scan_learned(eng: Engine, inp: Input, prop: Proposal, conw_r: float, n_from:
     screened = np.zeros(n, dtype=ROW_DTYPE)
     judged = np.zeros(n, dtype=np.int32)
     nf = C.c_longlong()
-    opts = PsLearnedOpts(block, n_candidates, halo, 1 if mirror else 0, judge._h if judge is not None else None)
+    opts = PsLearnedOpts(block, n_candidates, halo, 1 if mirror else 0, refine,
+                         judge._h if judge is not None else None)
     eng._check(lib.ps_scan_learned(eng._h, C.byref(inp.problem), C.byref(prop.c_net), C.byref(opts),
                                    inp.freq_start, inp.freq_step, n_from, n_to, conw_r, rows.ctypes.data,
                                    judged.ctypes.data_as(C.POINTER(C.c_int)), screened.ctypes.data, C.byref(nf)),

