@@ -182,6 +182,20 @@ int ps_scan(ps_engine *e, double freq_start, double freq_step, int n_from, int n
  * into the problem's pole table). */
 int ps_fit_batch(ps_engine *e, const double *freq, const int *pole, int n, double conw_r, ps_fit_out *out);
 
+/* ps_fit_batch for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  freq, pole and out are DEVICE pointers
+ * (caller-owned, n entries each, same meaning as in ps_fit_batch).  The fits'
+ * slot memory, slot states and counters live in one fixed 64 MiB array per
+ * device that the library holds until the process ends; the call allocates
+ * nothing (no hipMalloc/hipFree, no graph, no timing events) and clears what
+ * must start at zero with kernels.  Fewer slots fit there than the engine
+ * normally uses, so large batches run in more refill rounds; results are
+ * bitwise those of ps_fit_batch.  PS_ERR_LIMIT if one fit's slot memory
+ * exceeds the array, PS_ERR_ARG on a pole index out of range (checked on the
+ * device).  Calls that overlap on one device -- from any engine on it --
+ * share the array and race: serialise them. */
+int ps_fit_batch_noalloc(ps_engine *e, const double *freq, const int *pole, int n, double conw_r, ps_fit_out *out);
 /* Test seam: one normal-equation build (mrqcof) at each of n parameter
  * vectors cg [n*ma] (ma = ncoef+8, reference order).  alpha [n*mfit*mfit]
  * (full symmetric), beta [n*mfit], chisq [n]; chck [n*3] = sum Area*Nor. */

@@ -130,6 +130,7 @@ def _load() -> C.CDLL:
         "ps_set_progress": (C.c_int, [vp, PROGRESS_FN, vp]),
         "ps_aspect_features": (C.c_int, [C.POINTER(PsProblem), _fp]),
         "ps_rows_from_fits": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+        "ps_fit_batch_noalloc": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, vp]),
         "ps_scan_learned": (C.c_int, [vp, C.POINTER(PsProblem), C.POINTER(PsProposalNet), C.POINTER(PsLearnedOpts),
                                       C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, vp, _ip, vp,
                                       C.POINTER(C.c_longlong)]),
@@ -172,7 +173,7 @@ EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_
             "ps_input_scalars", "ps_format_row", "ps_shape_basis", "ps_shape_basis_many", "ps_input_read_deferred",
             "ps_input_shape_job", "ps_best_row", "ps_read_rows", "ps_input_read_ex", "ps_state_read",
             "ps_state_write", "ps_precalc_resume", "ps_set_progress", "ps_aspect_features", "ps_scan_learned",
-            "ps_rows_from_fits",
+            "ps_rows_from_fits", "ps_fit_batch_noalloc",
             "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
             "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
             "ps_mlp_train_step")

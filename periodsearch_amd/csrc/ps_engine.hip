@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1896,6 +1897,22 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevP
   (void)atry;
 }
 
+// Device memory of ps_fit_batch_noalloc: counters, slot states and slot
+// memory of its fits, in one fixed array per device, held until the process
+// ends.  Host code never passes the array's name to a kernel (in host code it
+// is only a stand-in for the symbol): it uses the device address that
+// hipGetSymbolAddress returns at ps_create (ps_engine::scratch_dev).
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr size_t kScratchHeader = 256;  // counters [3 * kMaxGroups], the pole check flag at byte 128
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void clear_flag(int *flag) { *flag = 0; }
+
+__global__ void check_poles(const int *pole, int n, int n_poles, int *flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (pole[i] < 0 || pole[i] >= n_poles)) atomicOr(flag, 1);
+}
+
 __global__ void reset_batch(SlotState *st, int nslots, int *queue, int *done) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nslots) {
@@ -2166,6 +2183,7 @@ struct ps_engine {
   long long last_items = 0, last_iters = 0;
   // optional per-launch timing of the model kernel (HIP events on the
   // engine's stream)
+  unsigned char *scratch_dev = nullptr;  // device address of this device's `scratch` array
   bool profile = false;
   // progress callback (ps_set_progress), called from the host poll loop
   ps_progress_fn progress = nullptr;
@@ -2321,7 +2339,7 @@ int ensure_slots(ps_engine *e, int nslots) {
 
 // Run W.n_items items through the slots until all are done.
 // Slot state, slot memory and counters taken from caller-provided device
-// memory instead of the engine's own allocations.
+// memory instead of the engine's own allocations (ps_fit_batch_noalloc).
 struct FixedSlots {
   SlotState *st;
   double *mem;
@@ -2799,6 +2817,24 @@ int ps_create(int device, ps_engine **out) {
     return PS_ERR_HIP;
   }
   e->cu_count = prop.multiProcessorCount;
+  // the device address of the no-alloc scratch array, looked up once per
+  // device (with that device current) and kept for every engine on it
+  {
+    static std::mutex mu;
+    static std::vector<unsigned char *> per_device;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)per_device.size() <= device) per_device.resize(device + 1, nullptr);
+    if (!per_device[device]) {
+      void *ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        delete e;
+        return PS_ERR_HIP;
+      }
+      per_device[device] = static_cast<unsigned char *>(ptr);
+    }
+    e->scratch_dev = per_device[device];
+  }
   if (const char *gr = getenv("PSAMD_GRAPH")) e->use_graph = atoi(gr) != 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
@@ -3023,6 +3059,56 @@ int ps_fit_batch(ps_engine *e, const double *freq, const int *pole, int n, doubl
   W.out = e->d_out;
   if ((rc = run_batch(e, W))) return rc;
   HIP_TRY(hipMemcpyAsync(out, e->d_out, n * sizeof(ps_fit_out), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return PS_OK;
+}
+
+int ps_fit_batch_noalloc(ps_engine *e, const double *freq, const int *pole, int n, double conw_r, ps_fit_out *out) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!freq || !pole || !out))) return PS_ERR_ARG;
+  if (n == 0) return PS_OK;
+  if (!e->scratch_dev) {
+    e->err = "no scratch memory on this device";
+    return PS_ERR_STATE;
+  }
+  const long long sd = slot_mem(e->G.ma, e->G.mfit, e->G.npad).total;
+  const size_t st_bytes = (sizeof(SlotState) + 255) / 256 * 256;  // per slot, state rounded up
+  const size_t per_slot = st_bytes + (size_t)sd * sizeof(double);
+  const long long fit_slots = (long long)((kScratchBytes - kScratchHeader) / per_slot);
+  if (fit_slots < 1) {
+    e->err = "ps_fit_batch_noalloc: one fit of this problem needs " + std::to_string(per_slot >> 10) +
+             " KiB of slot memory, more than the " + std::to_string(kScratchBytes >> 20) + " MiB scratch holds";
+    return PS_ERR_LIMIT;
+  }
+  const int max_slots = (int)std::min<long long>(fit_slots, e->max_slots);
+  unsigned char *base = e->scratch_dev;
+  int *counters = reinterpret_cast<int *>(base);
+  int *flag = reinterpret_cast<int *>(base + 128);
+  FixedSlots fx;
+  fx.counters = counters;
+  fx.st = reinterpret_cast<SlotState *>(base + kScratchHeader);
+  fx.mem = reinterpret_cast<double *>(base + kScratchHeader + (size_t)max_slots * st_bytes);
+  fx.max_slots = max_slots;
+  fx.slot_doubles = sd;
+  // pole indices are read by the kernels: check them on the device first
+  hipLaunchKernelGGL(clear_flag, dim3(1), dim3(1), 0, e->stream, flag);
+  hipLaunchKernelGGL(check_poles, dim3((n + 255) / 256), dim3(256), 0, e->stream, pole, n, e->n_poles, flag);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(e->host_done, flag, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->host_done[0]) {
+    e->err = "pole index out of range";
+    return PS_ERR_ARG;
+  }
+  DevWork W{};
+  W.n_items = n;
+  W.mode = 0;
+  W.freq = freq;
+  W.pole = pole;
+  W.conw_r = conw_r;
+  W.out = out;
+  if ((rc = run_batch(e, W, &fx))) return rc;
   HIP_TRY(hipStreamSynchronize(e->stream));
   return PS_OK;
 }
