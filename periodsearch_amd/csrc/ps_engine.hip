@@ -1062,6 +1062,141 @@ __device__ __forceinline__ int gauss_solve_wave(double (&a)[R], int n, double *x
   return 0;
 }
 
+// gauss_errc (period_search/gauss_errc.c:9-73) for 64 <= n <= 127 by four
+// waves with the augmented matrix [A|b] in registers: lane l holds columns l
+// (a0) and l + 64 (a1); wave w holds the physical rows p = w + 4 r, r <
+// RB.  As in gauss_solve_wave rows never move (a logical-row permutation,
+// replicated in every wave: perm/inv in two registers per lane, logical or
+// physical index l and l + 64), candidates and ties are judged by the
+// logical row-major index j*n + c, and every value b depends on is computed
+// with the reference's operations:
+//   pr = a[icol][c] * pivinv; a[j][c] = a[j][c] - pr * a[j][icol].
+// Per pivot step the wave holding the pivot row writes it to LDS (prow,
+// n + 1 doubles), a barrier, every wave eliminates its rows and reduces its
+// next-pivot candidate (red, two doubles per wave), a barrier.  Returns 0,
+// 1 (singular) or 2 (zero pivot), uniform over the workgroup; on 0 the
+// solution is in x[0..n) (written by the waves owning the rows).
+constexpr int kBlockSolveMax = 127;
+template <int RB>
+__device__ __forceinline__ int gauss_solve_block(double (&a0)[RB], double (&a1)[RB], int n, double *x,
+                                                 double *prow, double *red) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = lane, c1 = lane + 64;
+  int perm0 = c0, perm1 = c1, inv0 = c0, inv1 = c1;  // logical -> physical and physical -> logical
+  unsigned long long done0 = 0ull, done1 = 0ull;        // pivoted columns (= pivoted logical rows)
+  auto is_done = [&](int k) { return ((k < 64 ? done0 : done1) >> (k & 63)) & 1ull; };
+  auto perm_of = [&](int L) {  // wave-uniform L
+    return L < 64 ? __builtin_amdgcn_readlane(perm0, L) : __builtin_amdgcn_readlane(perm1, L - 64);
+  };
+  auto inv_of = [&](int p) { return p < 64 ? __builtin_amdgcn_readlane(inv0, p) : __builtin_amdgcn_readlane(inv1, p - 64); };
+  auto meet = [&](double bv, int be, double &ov, int &oe) {
+    // the waves' candidates meet in LDS; every thread ends with the same pair
+    wave_pivot_max(bv, be);
+    if (lane == 0) {
+      red[2 * wave] = bv;
+      red[2 * wave + 1] = (double)be;
+    }
+    __syncthreads();
+    ov = red[0];
+    oe = (int)red[1];
+#pragma unroll
+    for (int w = 1; w < kWaves; w++) pivot_max(ov, oe, red[2 * w], (int)red[2 * w + 1]);
+  };
+  double bv = -1.0;
+  int be = -1;
+#pragma unroll
+  for (int r = 0; r < RB; r++) {  // physical = logical order here
+    const int p = wave + 4 * r;
+    const double v0 = fabs(a0[r]), v1 = fabs(a1[r]);
+    const bool t0 = (p < n) & (c0 < n) & (v0 >= bv);
+    bv = t0 ? v0 : bv;
+    be = t0 ? p * n + c0 : be;
+    const bool t1 = (p < n) & (c1 < n) & (v1 > bv || (v1 == bv && p * n + c1 > be));
+    bv = t1 ? v1 : bv;
+    be = t1 ? p * n + c1 : be;
+  }
+  meet(bv, be, bv, be);
+  int prev_irow = -1, prev_icol = -1;
+  for (int i = 0; i < n; i++) {
+    int irow, icol;
+    if (be < 0) {  // no candidate (NaN matrix): the reference keeps the previous pivot
+      if (prev_icol < 0) return 1;
+      irow = prev_irow;
+      icol = prev_icol;
+    } else {
+      irow = be / n;
+      icol = be - irow * n;
+    }
+    if (is_done(icol)) return 1;  // column pivoted twice: the next scan reports singular
+    prev_irow = irow;
+    prev_icol = icol;
+    const int phr = perm_of(irow), phc = perm_of(icol);
+    if (irow != icol) {
+      perm0 = (c0 == irow) ? phc : ((c0 == icol) ? phr : perm0);
+      perm1 = (c1 == irow) ? phc : ((c1 == icol) ? phr : perm1);
+      inv0 = (c0 == phr) ? icol : ((c0 == phc) ? irow : inv0);
+      inv1 = (c1 == phr) ? icol : ((c1 == phc) ? irow : inv1);
+    }
+    const int pp = phr;  // physical row of the pivot row (logical row icol after the swap)
+    if (wave == (pp & 3)) {
+      const int rp = pp >> 2;
+      double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+      for (int r = 0; r < RB; r++) {
+        v0 = (r == rp) ? a0[r] : v0;
+        v1 = (r == rp) ? a1[r] : v1;
+      }
+      if (c0 <= n) prow[c0] = v0;
+      if (c1 <= n) prow[c1] = v1;
+    }
+    __syncthreads();
+    const double piv = prow[icol];
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    if (icol < 64)
+      done0 |= 1ull << icol;
+    else
+      done1 |= 1ull << (icol - 64);
+    const double pr0 = prow[min(c0, n)] * pivinv, pr1 = prow[min(c1, n)] * pivinv;  // new a[icol][c]
+    const bool cand0 = (c0 < n) && !is_done(c0), cand1 = (c1 < n) && !is_done(c1);
+    const int hi = icol >= 64, il = icol & 63;
+    bv = -1.0;
+    be = -1;
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int p = wave + 4 * r;
+      const double dum = lane_value(hi ? a1[r] : a0[r], il);
+      const double v0 = (p == pp) ? pr0 : a0[r] - pr0 * dum;
+      const double v1 = (p == pp) ? pr1 : a1[r] - pr1 * dum;
+      a0[r] = v0;
+      a1[r] = v1;
+      const int j = inv_of(min(p, 127));
+      const bool rowc = (p < n) && !is_done(j);
+      const double f0 = fabs(v0), f1 = fabs(v1);
+      const int i0 = j * n + c0, i1 = j * n + c1;
+      const bool t0 = rowc & cand0 & ((f0 > bv) | ((f0 == bv) & (i0 > be)));
+      bv = t0 ? f0 : bv;
+      be = t0 ? i0 : be;
+      const bool t1 = rowc & cand1 & ((f1 > bv) | ((f1 == bv) & (i1 > be)));
+      bv = t1 ? f1 : bv;
+      be = t1 ? i1 : be;
+    }
+    meet(bv, be, bv, be);
+  }
+  const int bh = n >= 64, bl = n & 63;  // column n (b)
+#pragma unroll
+  for (int r = 0; r < RB; r++) {
+    const int p = wave + 4 * r;
+    if (p < n) {
+      const double bp = lane_value(bh ? a1[r] : a0[r], bl);
+      const int j = inv_of(p);
+      if (lane == 0) x[j] = bp;
+    }
+  }
+  return 0;
+}
+
 // Problem of a slot.  One problem (MULTI false): the kernel argument P0, as
 // constant as any other argument.  Several (MULTI true): the slot's entry of
 // the device array of problems.
@@ -1086,13 +1221,17 @@ __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, 
 // LDS: the solver's two augmented matrices and pivot exchange; the final
 // curv of a finished fit (area/g [F]) reuses the same space.
 // WAVE (mfit <= 63): one wave per slot, the solve in registers
-// (gauss_solve_wave); otherwise 256 threads and the LDS solver.
+// (gauss_solve_wave); BLOCK (64 <= mfit <= 127): 256 threads, the solve in
+// registers of four waves (gauss_solve_block); otherwise 256 threads and the
+// LDS solver.
 constexpr int kWaveSolveMax = 63;
-template <bool MULTI, int RW>  // RW: register rows of the one-wave solver (>= mfit), 0 = LDS solver
-__global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : 4) step_kernel(DevProblem P0,
+template <bool MULTI, int RW, int BR = 0>  // RW: register rows of the one-wave solver (>= mfit); BR: rows per
+                                           // wave of the four-wave solver (>= mfit / 4); both 0: LDS solver
+__global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) step_kernel(DevProblem P0,
                                                                         const DevProblem *__restrict__ probs,
                                                                         DevWork W) {
   constexpr bool WAVE = RW > 0;
+  constexpr bool BLOCK = BR > 0;
   constexpr int kStepThreads = WAVE ? 64 : kThreads;
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
@@ -1213,6 +1352,24 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : 4) step_kernel(De
       am[p] = av * ((p == lane) ? (1 + lam) : 1.0);  // times 1.0 is exact; beta's lane never matches
     }
     err = gauss_solve_wave(am, mfit, da);
+    __syncthreads();
+  } else if constexpr (BLOCK) {
+#pragma clang fp contract(off)
+    // lane l loads columns l and l + 64 of the damped [alpha | beta] for the
+    // wave's rows p = wave + 4 r (column mfit is beta)
+    const double lam = S.lam;
+    const int lane = tid & 63, wave = tid >> 6;
+    double a0[BLOCK ? BR : 1], a1[BLOCK ? BR : 1];
+#pragma unroll
+    for (int r = 0; r < BR; r++) {
+      const int p = wave + 4 * r, j = min(p, mfit - 1);
+      const int ca = lane, cb = lane + 64;
+      const double va = (ca == mfit) ? bacc[j] : aacc[j * mfit + min(ca, mfit - 1)];
+      const double vb = (cb == mfit) ? bacc[j] : aacc[j * mfit + min(cb, mfit - 1)];
+      a0[r] = va * ((p == ca) ? (1 + lam) : 1.0);  // times 1.0 is exact; beta's column never matches
+      a1[r] = vb * ((p == cb) ? (1 + lam) : 1.0);
+    }
+    err = gauss_solve_block<BLOCK ? BR : 1>(a0, a1, mfit, da, sm, sm + 128);
     __syncthreads();
   } else {
   {
@@ -2061,7 +2218,11 @@ size_t mfma_lds(int nfac, int ncoef) {
   return d * sizeof(double) + kTilePts * sizeof(PointGeom);
 }
 
-size_t step_lds(int mfit, int nfac) {  // mfit = 0: the register solver (no LDS for the solve)
+size_t step_lds(int mfit, int nfac) {  // mfit = 0: a register solver (at most 136 doubles of LDS)
+  if (mfit == 0) {
+    const size_t fin = 2 * (size_t)nfac, reg = 128 + 2 * kWaves;
+    return (fin > reg ? fin : reg) * sizeof(double);
+  }
   const size_t copies = solve_in_place(mfit) ? 1 : 2;
   const size_t solve = mfit ? copies * (size_t)mfit * (mfit + 1) + 4 * kWaves : 0, fin = 2 * (size_t)nfac;
   return (solve > fin ? solve : fin) * sizeof(double);
@@ -2079,7 +2240,12 @@ using StepFn = void (*)(DevProblem, const DevProblem *, DevWork);
 // the smallest row count that holds them, or (mfit = 0 here) the LDS solver.
 template <bool MULTI>
 StepFn pick_step(int mfit) {
-  if (mfit <= 0) return step_kernel<MULTI, 0>;
+  if (mfit <= 0 || mfit > kBlockSolveMax) return step_kernel<MULTI, 0>;
+  if (mfit > kWaveSolveMax) {
+    if (mfit <= 64) return step_kernel<MULTI, 0, 16>;
+    if (mfit <= 96) return step_kernel<MULTI, 0, 24>;
+    return step_kernel<MULTI, 0, 32>;
+  }
   if (mfit <= 16) return step_kernel<MULTI, 16>;
   if (mfit <= 24) return step_kernel<MULTI, 24>;
   if (mfit <= 32) return step_kernel<MULTI, 32>;
@@ -2121,7 +2287,9 @@ struct KernelSet {
 inline long long size_class(int ncoef, int mfit) {
   const int nb = (ncoef + 15) / 16;
   const int fb = (mfit + 15) / 16, nt = (fb * (fb + 1) / 2 + 3) / 4;
-  const int rw = mfit > kWaveSolveMax ? 0 : (mfit <= 16 ? 16 : (mfit + 7) / 8 * 8);
+  const int rw = mfit > kBlockSolveMax ? 0
+                 : (mfit > kWaveSolveMax ? (mfit <= 64 ? 116 : (mfit <= 96 ? 124 : 132))
+                                         : (mfit <= 16 ? 16 : (mfit + 7) / 8 * 8));
   return ((long long)nb * 64 + nt) * 128 + rw;
 }
 
@@ -2243,8 +2411,8 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
     e->err = "no model kernel for this coefficient count";
     return PS_ERR_LIMIT;
   }
-  const bool wave_solve = mfit <= kWaveSolveMax;
-  ks.lds_step = step_lds(wave_solve ? 0 : mfit, F);
+  const bool wave_solve = mfit <= kWaveSolveMax, reg_solve = mfit <= kBlockSolveMax;
+  ks.lds_step = step_lds(reg_solve ? 0 : mfit, F);
   ks.lds_model = mixed ? mixed_layout(F, nb, mode).total : (ks.use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
   ks.lds_neq = neq_lds(mfit, mixed);
   const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
@@ -2252,7 +2420,7 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;
   }
-  ks.step = multi ? pick_step<true>(wave_solve ? mfit : 0) : pick_step<false>(wave_solve ? mfit : 0);
+  ks.step = multi ? pick_step<true>(mfit) : pick_step<false>(mfit);
   ks.step_threads = wave_solve ? 64 : kThreads;
   ks.curv = multi ? curv_kernel<true> : curv_kernel<false>;
   ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit) : pick_neq<true, false>(mfit))
