@@ -9,15 +9,16 @@
 //
 //   step_kernel   one workgroup per slot: finishes a converged fit and takes
 //                 the next fit from the work queue, or damps the accepted
-//                 normal equations and solves them in LDS (Gauss-Jordan with
-//                 full pivoting, period_search/gauss_errc.c:9-73) to form
-//                 the trial parameters (period_search/mrqmin.c:53-79).
-//   model_kernel  one workgroup per (slot, tile of 256 points): curv, the
+//                 normal equations and solves them (Gauss-Jordan with full
+//                 pivoting, period_search/gauss_errc.c:9-73: in the
+//                 registers of one wave up to 63 free parameters, of four
+//                 waves up to 127, in LDS beyond) to form the trial
+//                 parameters (period_search/mrqmin.c:53-79), and evaluates
+//                 the shape at them (curv.c).
+//   model kernel  one workgroup per (slot, tile of 64 points): the
 //                 per-point rotation/phase geometry in FP64 and the facet
-//                 loop with the brightness-derivative contraction
-//                 (period_search/bright.c, curv.c, matrix.c, phasec.c).
-//                 Facet normals and the Dsph basis are read-only tables
-//                 shared by every fit, read through the scalar cache.
+//                 loop with the brightness-derivative contraction on the
+//                 matrix cores (period_search/bright.c, matrix.c, phasec.c).
 //   neq_kernel    one workgroup per slot: convexity rows (conv.c), relative
 //                 renormalisation and the normal equations (mrqcof.c), then
 //                 accept/reject (mrqmin.c:82-107) and the driver bookkeeping
@@ -138,7 +139,7 @@ struct SlotMem {
 __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
   SlotMem m;
   long long o = 0;
-  m.area = o; o += kMaxFacets;   // curv() at atry: Area, g (written by curv_kernel)
+  m.area = o; o += kMaxFacets;   // curv() at atry: Area, g (written by step_kernel)
   m.g = o; o += kMaxFacets;
   m.J = o; o += (long long)(ma + 1) * npad;
   m.alpha0 = o; o += (long long)mfit * mfit;
@@ -199,6 +200,13 @@ __device__ void eval_curv(const DevProblem &P, const double *a, double *area, do
     area[f] = P.darea[f] * gf;
   }
 }
+
+// curv() at a slot's trial parameters (period_search/curv.c:10-38, the
+// reference's operation order) by the threads of one workgroup: Area and g
+// of every facet into the slot's memory, read by the model and normal-
+// equation kernels.  The slot memory layout (SlotMem) is passed in.
+struct SlotMem;
+__device__ void slot_curv(const DevProblem &P, const SlotMem &M, double *mem, int tid, int nthreads);
 
 // blmatrix (period_search/blmatrix.c:9-46) into LDS: B[9], dB/dbeta[9], dB/dlambda[9].
 __device__ void eval_blmatrix(double bet, double lam, double *blm) {
@@ -1197,6 +1205,16 @@ __device__ __forceinline__ int gauss_solve_block(double (&a0)[RB], double (&a1)[
   return 0;
 }
 
+__device__ void slot_curv(const DevProblem &P, const SlotMem &M, double *mem, int tid, int nthreads) {
+#pragma clang fp contract(off)
+  const double *atry = mem + M.atry;
+  for (int f = tid; f < P.nfac; f += nthreads) {
+    const double gf = curv_g(P, atry, f);
+    mem[M.g + f] = gf;
+    mem[M.area + f] = P.darea[f] * gf;
+  }
+}
+
 // Problem of a slot.  One problem (MULTI false): the kernel argument P0, as
 // constant as any other argument.  Several (MULTI true): the slot's entry of
 // the device array of problems.
@@ -1217,7 +1235,7 @@ __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, 
 // step_kernel: one workgroup per slot.  Finishes a fit whose loop predicate
 // is false (per-fit summary) and takes the next fit from the queue, or damps
 // the accepted normal equations and solves them, forming the trial
-// parameters (their curv() runs in curv_kernel).
+// parameters and their shape (curv()).
 // LDS: the solver's two augmented matrices and pivot exchange; the final
 // curv of a finished fit (area/g [F]) reuses the same space.
 // WAVE (mfit <= 63): one wave per slot, the solve in registers
@@ -1325,6 +1343,8 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) ste
       S.eval = 1;
       S.err = 0;
     }
+    __syncthreads();
+    slot_curv(P, M, mem, tid, kStepThreads);
     return;
   }
   // damped normal equations and solve (mrqmin.c:53-79)
@@ -1410,30 +1430,13 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) ste
       const int l = P.fidx[jf];
       atry[l] = a[l] + da[jf];
     }
+    __syncthreads();
+    slot_curv(P, M, mem, tid, kStepThreads);  // the shape at the trial parameters
   }
   if (tid == 0) {
     S.eval = err == 0 ? 1 : 0;
     S.err = err;
   }
-}
-
-// curv_kernel: curv() at the trial parameters of every evaluating slot,
-// one thread per facet (period_search/curv.c:10-38, reference operation
-// order); Area and g go to the slot for the model and normal-equation
-// kernels.  blockIdx.x = chunk * nslots + slot.
-template <bool MULTI>
-__global__ void __launch_bounds__(kThreads) curv_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
-#pragma clang fp contract(off)
-  const int slot = blockIdx.x % W.nslots, chunk = blockIdx.x / W.nslots;
-  if (!W.st[slot].eval) return;
-  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
-  const int f = chunk * kThreads + threadIdx.x;
-  if (f >= P.nfac) return;
-  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
-  double *mem = W.mem + (size_t)slot * W.slot_doubles;
-  const double gf = curv_g(P, mem + M.atry, f);
-  mem[M.g + f] = gf;
-  mem[M.area + f] = P.darea[f] * gf;
 }
 
 // model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
@@ -2266,7 +2269,6 @@ NeqFn pick_neq(int mfit) {
   return nullptr;
 }
 
-using CurvFn = void (*)(DevProblem, const DevProblem *, DevWork);
 
 // The kernels of one LM iteration, chosen for a set of problems: their
 // largest coefficient count and free-parameter count pick the instances,
@@ -2275,7 +2277,6 @@ struct KernelSet {
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
   StepFn step = nullptr;
-  CurvFn curv = nullptr;
   size_t lds_step = 0, lds_model = 0, lds_neq = 0;
   int step_threads = kThreads;
   bool use_mfma = true;
@@ -2329,7 +2330,6 @@ struct ps_engine {
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
   void (*step)(DevProblem, const DevProblem *, DevWork) = nullptr;
-  void (*curv)(DevProblem, const DevProblem *, DevWork) = nullptr;
   bool use_mfma = true;   // FP64: matrix-core contraction (default) or the VALU kernel
   // slot memory
   SlotState *st = nullptr;
@@ -2422,7 +2422,6 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   }
   ks.step = multi ? pick_step<true>(mfit) : pick_step<false>(mfit);
   ks.step_threads = wave_solve ? 64 : kThreads;
-  ks.curv = multi ? curv_kernel<true> : curv_kernel<false>;
   ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit) : pick_neq<true, false>(mfit))
                  : (multi ? pick_neq<false, true>(mfit) : pick_neq<false, false>(mfit));
   if (!ks.neq) {
@@ -2540,7 +2539,6 @@ int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
   eks.model = e->model;
   eks.neq = e->neq;
   eks.step = e->step;
-  eks.curv = e->curv;
   eks.lds_step = e->lds_step;
   eks.lds_model = e->lds_model;
   eks.lds_neq = e->lds_neq;
@@ -2624,8 +2622,6 @@ int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
         const KernelSet &K = *lane[l].ks;
         const int ns = Wk.nslots;
         hipLaunchKernelGGL(K.step, dim3(ns), dim3(K.step_threads), K.lds_step, sg[l], P0, probs, Wk);
-        hipLaunchKernelGGL(K.curv, dim3(ns * ((K.G.nfac + kThreads - 1) / kThreads)), dim3(kThreads), 0, sg[l], P0,
-                           probs, Wk);
         if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + l)], sg[l]));
         hipLaunchKernelGGL(K.model, dim3(ns * (K.use_mfma ? K.G.ntiles64 : K.G.ntiles)), dim3(kThreads),
                            K.lds_model, sg[l], P0, probs, Wk);
@@ -3143,7 +3139,6 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
   e->model = ks.model;
   e->neq = ks.neq;
   e->step = ks.step;
-  e->curv = ks.curv;
   e->lds_step = ks.lds_step;
   e->lds_model = ks.lds_model;
   e->lds_neq = ks.lds_neq;
