@@ -114,6 +114,7 @@ typedef struct ps_fit_out {
   double alamda;   /* final Marquardt lambda */
   int niter;       /* LM iterations run */
   int err;         /* last gauss_errc code (0 = none) */
+  double latency_us; /* device wall clock from the fit taking its slot to its summary [us] */
 } ps_fit_out;
 
 /* One output row: the six numbers of period_search_out. */

@@ -80,6 +80,16 @@ typedef struct ps_proposal_net {
   const float *x_mean, *x_std, *t_mean, *t_std;
 } ps_proposal_net;
 
+/* Load a trained network from an uncompressed .npz (numpy.savez) holding
+ * float32 1-D arrays params, x_mean, x_std, t_mean and t_std: n_in =
+ * len(x_mean), n_out = len(t_mean), nbins = n_in - 7, the hidden width from
+ * len(params) = (n_in + 1) H + (H + 1) H + (H + 1) n_out.  Creates the
+ * network on `device` for batches up to max_batch and fills *net, which owns
+ * the network and the scalings until ps_proposal_close.  PS_ERR_IO: the file
+ * is missing, compressed or not in that form. */
+int ps_proposal_open(const char *path, int device, int max_batch, ps_proposal_net *net);
+void ps_proposal_close(ps_proposal_net *net);
+
 /* How the learned scan screens and judges. */
 typedef struct ps_learned_opts {
   int block;         /* trial periods per screening batch */

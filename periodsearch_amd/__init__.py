@@ -64,7 +64,7 @@ class PsShapeJob(C.Structure):
 
 class PsFitOut(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dev", "chisq", "rchisq", "prd", "la", "be", "dark", "totarea", "alamda")] + [
-        ("niter", C.c_int), ("err", C.c_int)]
+        ("niter", C.c_int), ("err", C.c_int), ("latency_us", C.c_double)]
 
 
 class PsRow(C.Structure):
@@ -86,12 +86,15 @@ class PsState(C.Structure):
                 ("sum_dark", C.c_double), ("test_periods", C.c_int)]
 
 
+PS_OK, PS_ERR_ARG, PS_ERR_HIP, PS_ERR_LIMIT, PS_ERR_IO, PS_ERR_STATE = 0, -1, -2, -3, -4, -5
 PS_STATE_NONE, PS_STATE_BASE, PS_STATE_CUDA, PS_STATE_OC = 0, 1, 2, 3
 PS_READ_DEFER_SHAPE, PS_READ_CLAMP_MIN = 1, 2
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_longlong, C.c_longlong)
 
 FIT_DTYPE = np.dtype([(n, "f8") for n in ("dev", "chisq", "rchisq", "prd", "la", "be", "dark", "totarea", "alamda")]
-                     + [("niter", "i4"), ("err", "i4")])
+                     + [("niter", "i4"), ("err", "i4"), ("latency_us", "f8")])
+# the fields that a fit's arithmetic fixes (latency_us is a timing)
+FIT_RESULT_FIELDS = FIT_DTYPE.names[:-1]
 ROW_DTYPE = np.dtype([(n, "f8") for n in ("period_h", "rms", "chisq", "dark", "lambda", "beta")])
 assert FIT_DTYPE.itemsize == C.sizeof(PsFitOut)
 assert ROW_DTYPE.itemsize == C.sizeof(PsRow)
@@ -134,6 +137,8 @@ def _load() -> C.CDLL:
         "ps_scan_learned": (C.c_int, [vp, C.POINTER(PsProblem), C.POINTER(PsProposalNet), C.POINTER(PsLearnedOpts),
                                       C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, vp, _ip, vp,
                                       C.POINTER(C.c_longlong)]),
+        "ps_proposal_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(PsProposalNet)]),
+        "ps_proposal_close": (None, [C.POINTER(PsProposalNet)]),
         "ps_input_shape_job": (C.c_int, [vp, C.POINTER(PsShapeJob)]),
         "ps_shape_basis_many": (C.c_int, [vp, C.c_int, C.POINTER(PsShapeJob)]),
         "ps_input_free": (None, [vp]),
@@ -173,7 +178,7 @@ EXPORTED = ("ps_create", "ps_destroy", "ps_last_error", "ps_set_precision", "ps_
             "ps_input_scalars", "ps_format_row", "ps_shape_basis", "ps_shape_basis_many", "ps_input_read_deferred",
             "ps_input_shape_job", "ps_best_row", "ps_read_rows", "ps_input_read_ex", "ps_state_read",
             "ps_state_write", "ps_precalc_resume", "ps_set_progress", "ps_aspect_features", "ps_scan_learned",
-            "ps_rows_from_fits", "ps_fit_batch_noalloc",
+            "ps_rows_from_fits", "ps_fit_batch_noalloc", "ps_proposal_open", "ps_proposal_close",
             "ps_lc_features", "ps_fit_batch_from", "ps_mlp_create", "ps_mlp_destroy", "ps_mlp_last_error",
             "ps_mlp_param_count", "ps_mlp_set_params", "ps_mlp_get_params", "ps_mlp_forward", "ps_mlp_gradients",
             "ps_mlp_train_step")

@@ -126,6 +126,7 @@ struct DevProblem {
 struct SlotState {
   int item, phase, niter, err;
   int eval, sel, prob, pad1;  // prob: index of the fit's problem (asteroid) in the batch
+  long long t_start;          // device wall clock when the fit took the slot
   double lam, ochisq, chisq, chitr, dev_new, dev_old, iter_diff, rchisq;
   double chck[3], chtr[3];
 };
@@ -158,6 +159,7 @@ __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
 // One batch of work: fits (mode 0) or single model evaluations (mode 1).
 struct DevWork {
   int n_items, mode, nslots, pad;  // mode 0: fits from the reference starts, 1: evaluations, 2: fits from cg_in
+  double us_per_tick;       // period of the device wall clock [us]
   const double *freq;       // [n_items] (mode 0)
   const int *pole;          // [n_items] (mode 0)
   const double *cg_in;      // [n_items][ma] (modes 1, 2)
@@ -1305,6 +1307,7 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) ste
       o.alamda = S.lam;
       o.niter = S.niter;
       o.err = S.err;
+      o.latency_us = (double)(wall_clock64() - S.t_start) * W.us_per_tick;
       W.out[s_item] = o;
       atomicAdd(W.done, 1);
     }
@@ -1339,6 +1342,7 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) ste
     if (tid == 0) {
       S.item = item;
       S.prob = s_prob;
+      S.t_start = wall_clock64();
       S.phase = PH_INIT;
       S.eval = 1;
       S.err = 0;
@@ -2316,6 +2320,7 @@ struct ps_engine {
   size_t iprob_cap = 0, conw_cap = 0;
   std::vector<void *> prob_allocs;
   int n_poles = 0, cu_count = 0, max_slots = 2048;
+  double us_per_tick = 0;  // device wall clock period [us]
   float feat_alpha = 0, feat_aspect = 0;  // mean solar phase angle, mean |Earth latitude| [rad]
   float *d_feat = nullptr;
   size_t feat_cap = 0;
@@ -2517,6 +2522,7 @@ struct FixedSlots {
 
 int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
   if (W.n_items <= 0) return PS_OK;
+  W.us_per_tick = e->us_per_tick;
   // as few refill rounds as max_slots allows, with the items spread evenly
   // over them (fits of equal length then finish together, no thin tail)
   const int max_slots = fixed ? fixed->max_slots : e->max_slots;
@@ -2981,6 +2987,12 @@ int ps_create(int device, ps_engine **out) {
     return PS_ERR_HIP;
   }
   e->cu_count = prop.multiProcessorCount;
+  int wclk_khz = 0;
+  if (hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || wclk_khz <= 0) {
+    delete e;
+    return PS_ERR_HIP;
+  }
+  e->us_per_tick = 1e3 / wclk_khz;
   // the device address of the no-alloc scratch array, looked up once per
   // device (with that device current) and kept for every engine on it
   {

@@ -24,6 +24,8 @@
 // results in the reference's sense.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -118,6 +120,81 @@ std::vector<int> pick_candidates(const ps_row *rows, int n, int n_candidates, in
   return out;
 }
 
+// ---- network file: uncompressed .npz of 1-D float32 .npy members --------
+// A zip local header (signature 50 4b 03 04) is followed by the member's
+// name, an extra field (zip64 sizes under tag 0x0001 when the 32-bit sizes
+// read 0xffffffff) and the stored bytes; an .npy member is the magic
+// \x93NUMPY, a version, a header length (2 bytes in version 1, 4 after) and
+// a dict text with 'descr', 'fortran_order' and 'shape'.
+bool read_file(const char *path, std::vector<unsigned char> &buf) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return false;
+  unsigned char tmp[1 << 16];
+  size_t n;
+  while ((n = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+  const bool ok = !ferror(f);
+  fclose(f);
+  return ok;
+}
+
+unsigned long long le(const unsigned char *p, int n) {
+  unsigned long long v = 0;
+  for (int i = n - 1; i >= 0; i--) v = (v << 8) | p[i];
+  return v;
+}
+
+// The float32 values of a 1-D little-endian .npy member; false if not that.
+bool npy_f32(const unsigned char *d, size_t size, std::vector<float> &out) {
+  if (size < 10 || memcmp(d, "\x93NUMPY", 6) != 0) return false;
+  const int major = d[6];
+  const size_t hl = major == 1 ? le(d + 8, 2) : le(d + 8, 4);
+  const size_t h0 = major == 1 ? 10 : 12;
+  if (h0 + hl > size) return false;
+  const std::string h((const char *)d + h0, hl);
+  if (h.find("'descr': '<f4'") == std::string::npos) return false;
+  const size_t sp = h.find("'shape': (");
+  if (sp == std::string::npos) return false;
+  const size_t close = h.find(')', sp);
+  const std::string dims = h.substr(sp + 10, close - sp - 10);
+  char *end = nullptr;
+  const long long n = strtoll(dims.c_str(), &end, 10);
+  for (const char *c = end; *c; c++)  // exactly one dimension: "N," or "N"
+    if (*c != ',' && *c != ' ') return false;
+  if (n < 0 || h0 + hl + (size_t)n * 4 != size) return false;
+  out.resize(n);
+  memcpy(out.data(), d + h0 + hl, (size_t)n * 4);
+  return true;
+}
+
+bool npz_members(const std::vector<unsigned char> &z, std::vector<std::pair<std::string, std::vector<float>>> &m) {
+  size_t off = 0;
+  while (off + 30 <= z.size() && le(&z[off], 4) == 0x04034b50) {
+    const unsigned char *h = &z[off];
+    const int flags = (int)le(h + 6, 2), method = (int)le(h + 8, 2);
+    unsigned long long csize = le(h + 18, 4), usize = le(h + 22, 4);
+    const size_t nl = le(h + 26, 2), el = le(h + 28, 2);
+    if (off + 30 + nl + el > z.size()) return false;
+    std::string name((const char *)h + 30, nl);
+    const unsigned char *ex = h + 30 + nl;
+    for (size_t q = 0; q + 4 <= el;) {
+      const size_t tag = le(ex + q, 2), len = le(ex + q + 2, 2);
+      if (tag == 1 && len >= 16) {
+        if (usize == 0xffffffffULL) usize = le(ex + q + 4, 8);
+        if (csize == 0xffffffffULL) csize = le(ex + q + 12, 8);
+      }
+      q += 4 + len;
+    }
+    if (method != 0 || (flags & 8) || csize != usize) return false;  // stored members only
+    const size_t d0 = off + 30 + nl + el;
+    if (d0 + csize > z.size()) return false;
+    if (name.size() > 4 && name.compare(name.size() - 4, 4, ".npy") == 0) name.resize(name.size() - 4);
+    std::vector<float> v;
+    if (npy_f32(&z[d0], csize, v)) m.emplace_back(name, std::move(v));
+    off = d0 + csize;
+  }
+  return !m.empty();
+}
+
 }  // namespace
 
 extern "C" {
@@ -174,6 +251,61 @@ int ps_aspect_features(const ps_problem *p, float out[5]) {
     out[4] = (float)(n / 10.0);
   }
   return PS_OK;
+}
+
+int ps_proposal_open(const char *path, int device, int max_batch, ps_proposal_net *net) {
+  if (!path || !net || max_batch < 1) return PS_ERR_ARG;
+  memset(net, 0, sizeof *net);
+  std::vector<unsigned char> z;
+  std::vector<std::pair<std::string, std::vector<float>>> m;
+  if (!read_file(path, z) || !npz_members(z, m)) return PS_ERR_IO;
+  auto get = [&](const char *k) -> const std::vector<float> * {
+    for (auto &kv : m)
+      if (kv.first == k) return &kv.second;
+    return nullptr;
+  };
+  const std::vector<float> *par = get("params"), *xm = get("x_mean"), *xs = get("x_std"), *tm = get("t_mean"),
+                           *ts = get("t_std");
+  if (!par || !xm || !xs || !tm || !ts || xm->size() != xs->size() || tm->size() != ts->size()) return PS_ERR_IO;
+  const long long ni = (long long)xm->size(), no = (long long)tm->size(), np = (long long)par->size();
+  if (ni <= 7 || no != 12) return PS_ERR_IO;
+  // (ni + 1) H + (H + 1) H + (H + 1) no = np: H^2 + (ni + no + 2) H + no - np = 0
+  const double b = (double)(ni + no + 2), c = (double)(no - np);
+  const long long hid = llround((-b + std::sqrt(b * b - 4 * c)) / 2);
+  if (hid < 1 || (ni + 1) * hid + (hid + 1) * hid + (hid + 1) * no != np) return PS_ERR_IO;
+  ps_mlp *mlp = nullptr;
+  int rc = ps_mlp_create(device, (int)ni, (int)hid, (int)no, max_batch, &mlp);
+  if (rc != PS_OK) return rc;
+  if ((rc = ps_mlp_set_params(mlp, par->data(), np)) != PS_OK) {
+    ps_mlp_destroy(mlp);
+    return rc;
+  }
+  float *sc = (float *)malloc(sizeof(float) * 2 * (ni + no));
+  if (!sc) {
+    ps_mlp_destroy(mlp);
+    return PS_ERR_ARG;
+  }
+  memcpy(sc, xm->data(), ni * 4);
+  memcpy(sc + ni, xs->data(), ni * 4);
+  memcpy(sc + 2 * ni, tm->data(), no * 4);
+  memcpy(sc + 2 * ni + no, ts->data(), no * 4);
+  net->mlp = mlp;
+  net->n_in = (int)ni;
+  net->n_out = (int)no;
+  net->nbins = (int)ni - 7;
+  net->max_batch = max_batch;
+  net->x_mean = sc;
+  net->x_std = sc + ni;
+  net->t_mean = sc + 2 * ni;
+  net->t_std = sc + 2 * ni + no;
+  return PS_OK;
+}
+
+void ps_proposal_close(ps_proposal_net *net) {
+  if (!net) return;
+  if (net->mlp) ps_mlp_destroy(net->mlp);
+  free((void *)net->x_mean);  // the block that holds all four scalings
+  memset(net, 0, sizeof *net);
 }
 
 int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *net, const ps_learned_opts *opt,

@@ -5,6 +5,17 @@
 //   period_search_amd [--device N] [--in F] [--out F] [--state F] [--batch P]
 //                     [--round-pole] [--prec fp64|fp32|bf16|bf16x3|bf16x6]
 //                     [--state-format base|cuda|oc] [--clamp-min] [--progress]
+//                     [--learned [--net F] [--block B] [--candidates K] [--halo H]
+//                      [--refine R] [--mirror] [--judged-out F]]
+//
+// --learned runs the learned scan (ps_scan_learned) instead of the ten-pole
+// scan: every trial period fitted from the network's proposed start in the
+// --prec arithmetic, the candidates of each block and their neighbours
+// refitted from the ten poles by a second engine in FP64 (those rows are the
+// reference's; --judged-out writes "n judged" per row); the convexity-weight
+// precalc then runs on that FP64 engine.  The network defaults to
+// proposal_net.npz next to this executable.  In this mode --batch (the
+// rows per call and checkpoint) defaults to the whole period range.
 //
 // Files default to period_search_in / period_search_out / period_search_state
 // in the working directory (period_search_BOINC.cpp:81-83).  Checkpoints are
@@ -16,13 +27,16 @@
 // (period_search_BOINC.cpp:238-251).  An optimised-CPU-app checkpoint taken
 // inside a convexity-weight round resumes that round
 // (period_search_optimization_cpu/period_search/period_search_BOINC.cpp:627-640).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "periodsearch.h"
+#include "ps_proposal.h"
 
 namespace {
 
@@ -46,12 +60,24 @@ int parse_state_format(const std::string &s) {
   return -1;
 }
 
+// proposal_net.npz in the directory of this executable
+std::string default_net() {
+  char buf[4096];
+  const ssize_t n = readlink("/proc/self/exe", buf, sizeof buf - 1);
+  if (n <= 0) return "proposal_net.npz";
+  std::string p(buf, n);
+  const size_t sl = p.rfind('/');
+  return (sl == std::string::npos ? std::string(".") : p.substr(0, sl)) + "/proposal_net.npz";
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   int device = 0, batch = 512, round_pole = 0, prec = PS_PREC_FP64, state_format = PS_STATE_CUDA, flags = 0;
-  bool progress = false;
+  bool progress = false, learned = false, batch_given = false;
+  ps_learned_opts lopt{2240, 8, 1, 0, 8, nullptr};
   std::string in_path = "period_search_in", out_path = "period_search_out", state_path = "period_search_state";
+  std::string net_path = default_net(), judged_path;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     auto next = [&]() -> std::string {
@@ -71,6 +97,23 @@ int main(int argc, char **argv) {
       state_path = next();
     } else if (a == "--batch") {
       batch = atoi(next().c_str());
+      batch_given = true;
+    } else if (a == "--learned") {
+      learned = true;
+    } else if (a == "--net") {
+      net_path = next();
+    } else if (a == "--block") {
+      lopt.block = atoi(next().c_str());
+    } else if (a == "--candidates") {
+      lopt.n_candidates = atoi(next().c_str());
+    } else if (a == "--halo") {
+      lopt.halo = atoi(next().c_str());
+    } else if (a == "--refine") {
+      lopt.refine = atoi(next().c_str());
+    } else if (a == "--mirror") {
+      lopt.mirror = 1;
+    } else if (a == "--judged-out") {
+      judged_path = next();
     } else if (a == "--round-pole") {
       round_pole = 1;
     } else if (a == "--clamp-min") {
@@ -120,6 +163,23 @@ int main(int argc, char **argv) {
     return 2;
   }
   if (progress) ps_set_progress(eng, print_progress, nullptr);
+  // learned scan: the network and an FP64 engine for the ten-pole judge fits
+  ps_proposal_net net{};
+  ps_engine *judge = nullptr;
+  if (learned) {
+    if (!batch_given) batch = n_periods;
+    const int rc = ps_proposal_open(net_path.c_str(), device, std::max(1, std::min(batch, lopt.block)), &net);
+    if (rc != PS_OK) {
+      fprintf(stderr, "cannot load the proposal network %s (%d)\n", net_path.c_str(), rc);
+      return 2;
+    }
+    if (ps_create(device, &judge) != PS_OK || ps_set_precision(judge, PS_PREC_FP64) != PS_OK ||
+        ps_set_problem(judge, &prob) != PS_OK) {
+      fprintf(stderr, "judge engine setup failed: %s\n", judge ? ps_last_error(judge) : "no device");
+      return 2;
+    }
+    lopt.judge = judge;
+  }
   ps_state st;
   ps_state_read(state_path.c_str(), &st);
   const bool resume = st.format != PS_STATE_NONE;
@@ -151,11 +211,13 @@ int main(int argc, char **argv) {
       exit(1);
     }
   };
-  // convexity-weight precalc (period_search_BOINC.cpp:487-715)
+  // convexity-weight precalc (period_search_BOINC.cpp:487-715), in FP64 on
+  // the judge engine when the learned scan has one
+  ps_engine *pre = judge ? judge : eng;
   while (new_conw != 1 && conw_r * escl * escl < 10.0) {
-    if (ps_precalc_resume(eng, freq_start, freq_end, freq_step, done_periods, sum_dark, &conw_r, &new_conw) !=
+    if (ps_precalc_resume(pre, freq_start, freq_end, freq_step, done_periods, sum_dark, &conw_r, &new_conw) !=
         PS_OK) {
-      fprintf(stderr, "precalc failed: %s\n", ps_last_error(eng));
+      fprintf(stderr, "precalc failed: %s\n", ps_last_error(pre));
       return 2;
     }
     done_periods = 0;
@@ -167,12 +229,22 @@ int main(int argc, char **argv) {
     fprintf(stderr, "output open failed: %s\n", out_path.c_str());
     return 1;
   }
+  FILE *jout = nullptr;
+  if (learned && !judged_path.empty() && !(jout = fopen(judged_path.c_str(), n_start > 1 ? "a" : "w"))) {
+    fprintf(stderr, "output open failed: %s\n", judged_path.c_str());
+    return 1;
+  }
   std::vector<ps_row> rows(batch);
+  std::vector<int> judged(batch, 1);
   char buf[256];
   for (int n0 = n_start; n0 <= n_periods; n0 += batch) {
     int n1 = n0 + batch - 1;
     if (n1 > n_periods) n1 = n_periods;
-    if (ps_scan(eng, freq_start, freq_step, n0, n1, conw_r, rows.data()) != PS_OK) {
+    long long n_fits = 0;
+    const int rc = learned ? ps_scan_learned(eng, &prob, &net, &lopt, freq_start, freq_step, n0, n1, conw_r,
+                                             rows.data(), judged.data(), nullptr, &n_fits)
+                           : ps_scan(eng, freq_start, freq_step, n0, n1, conw_r, rows.data());
+    if (rc != PS_OK) {
       fprintf(stderr, "scan failed: %s\n", ps_last_error(eng));
       return 2;
     }
@@ -181,13 +253,20 @@ int main(int argc, char **argv) {
       if (n == 1) r.dark = conw_r * escl * escl;
       ps_format_row(&r, round_pole, buf, sizeof buf);
       fputs(buf, out);
+      if (jout) fprintf(jout, "%d %d\n", n, judged[n - n0]);
     }
     fflush(out);
+    if (jout) fflush(jout);
     new_conw = 1;
     checkpoint(n1, 0);
     fprintf(stderr, "rows %d..%d of %d done\n", n0, n1, n_periods);
   }
   fclose(out);
+  if (jout) fclose(jout);
+  if (learned) {
+    ps_proposal_close(&net);
+    ps_destroy(judge);
+  }
   ps_destroy(eng);
   ps_input_free(in);
   return 0;

@@ -1,7 +1,8 @@
 """Host-side driver pieces (no GPU): checkpoint formats, the optional
 DBL_MIN clamp of the input reader, the driver's argument checks, the
-NaN-safe pole reduction of the learned scan, and the reference's real
+network-file reader, the NaN-safe pole reduction of the learned scan, and the reference's real
 work-unit inputs parsed against the oracle."""
+import ctypes
 import glob
 import os
 import subprocess
@@ -85,6 +86,44 @@ def test_driver_rejects_unknown_options(tmp_path):
     for args in (["--prec", "bf16x7"], ["--state-format", "xml"], ["--bogus"], ["--prec"]):
         r = subprocess.run([ps.DRIVER_PATH, *args], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2, (args, r.stderr)
+
+
+# ---------------------------------------------------------------- network file
+def _open_net(path):
+    net = ps.PsProposalNet()
+    rc = ps.lib.ps_proposal_open(os.fsencode(str(path)), 0, 64, ctypes.byref(net))
+    if rc == 0:
+        ps.lib.ps_proposal_close(ctypes.byref(net))
+    return rc
+
+
+def test_network_file_reader(tmp_path):
+    """ps_proposal_open parses the stored (uncompressed) .npz that
+    proposal.py writes and refuses other files with PS_ERR_IO before it
+    touches a device; a well-formed file reaches the device step (which
+    fails here, with no GPU, with another code)."""
+    from periodsearch_amd.proposal import DEFAULT_NET
+
+    z = np.load(DEFAULT_NET, allow_pickle=False)
+    arrs = {k: z[k] for k in ("params", "x_mean", "x_std", "t_mean", "t_std")}
+    ok = tmp_path / "ok.npz"
+    np.savez(ok, **arrs)
+    assert _open_net(DEFAULT_NET) not in (ps.PS_ERR_IO, ps.PS_ERR_ARG)
+    assert _open_net(ok) not in (ps.PS_ERR_IO, ps.PS_ERR_ARG)
+    bad = {
+        "missing": None,
+        "compressed": lambda p: np.savez_compressed(p, **arrs),
+        "float64": lambda p: np.savez(p, **{**arrs, "params": arrs["params"].astype(np.float64)}),
+        "short_params": lambda p: np.savez(p, **{**arrs, "params": arrs["params"][:-1]}),
+        "no_t_std": lambda p: np.savez(p, **{k: v for k, v in arrs.items() if k != "t_std"}),
+        "two_dims": lambda p: np.savez(p, **{**arrs, "x_mean": arrs["x_mean"][None, :]}),
+        "not_zip": lambda p: open(p, "wb").write(b"\x93NUMPY" + bytes(100)),
+    }
+    for name, make in bad.items():
+        p = tmp_path / f"{name}.npz"
+        if make:
+            make(p)
+        assert _open_net(p) == ps.PS_ERR_IO, name
 
 
 # ---------------------------------------------------------------- pole reduction

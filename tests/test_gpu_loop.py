@@ -3,7 +3,8 @@ the convexity-weight precalc (reference-held row 1, and a case where the
 weight doubles seven times), the convergence stop rule (stop < 1 with a
 minimum iteration count), whole 50-iteration fits held to the reference
 arithmetic's own one-ulp spread, the work-unit shape row by row, the
-reference's real work-unit inputs, and the 64-asteroid uneven batch.
+reference's real work-unit inputs, the 64-asteroid uneven batch, and the
+per-fit latency the engine measures on the device clock.
 
 Bars:
   * precalc: the same final convexity weight; the row-1 column printed as in
@@ -173,6 +174,25 @@ def test_fp64_fits_within_the_reference_own_spread(small_incl_input):
     rel = _check_fits(out, orc, env, "small asteroid, 50 iterations")
     assert np.all(out["niter"] == 50)
     assert np.all(rel[env < 1e-10] < 1e-9)
+    eng.close()
+
+
+def test_fit_latency_is_measured_on_the_device(small_incl_input):
+    """ps_fit_out.latency_us: the device wall clock from a fit taking its
+    slot to its summary.  Every fit's is positive and no longer than the
+    batch's GPU time; 50-iteration fits run the same number of iterations,
+    so their latencies agree within a few times their median."""
+    inp, eng = _engine(small_incl_input)
+    freqs = [inp.freq_start - n * inp.freq_step for n in range(40) for _ in range(10)]
+    poles = [m for _ in range(40) for m in range(10)]
+    out = eng.fit_batch(freqs, poles, inp.conw_r0)
+    ms, n, _ = eng.last_batch()
+    lat = out["latency_us"]
+    print(f"{n} fits in {ms:.2f} ms: latency p50 {np.median(lat):.0f} us, max {lat.max():.0f} us")
+    assert n == len(out) == 400
+    assert np.all(np.isfinite(lat)) and np.all(lat > 0)
+    assert lat.max() <= ms * 1e3 * 1.01 + 50
+    assert lat.max() < 5 * np.median(lat)
     eng.close()
 
 
