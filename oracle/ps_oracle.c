@@ -64,6 +64,10 @@ struct orc_ctx {
   /* dev_new after iteration k of the last fit, k = 1..Niter (test record;
    * the reference keeps no such record) */
   double dev_trace[ORC_MAX_N_ITER + 2];
+  /* test switch, not in the reference: 1 rounds every derivative bright()
+   * and conv() return to float before mrqcof uses it (the precision of a
+   * Jacobian held in FP32) */
+  int dyda_f32;
 };
 
 orc_ctx *orc_create(void) { return (orc_ctx *)calloc(1, sizeof(orc_ctx)); }
@@ -598,6 +602,8 @@ static double mrqcof(orc_ctx *c, const double a[], const int ia[], int ma, doubl
         ymod = bright(c, xx1, xx2, c->tim[np], a, dyda, ma);
       else
         ymod = conv(c, jp, dyda, ma);
+      if (c->dyda_f32)
+        for (l = 1; l <= ma; l++) dyda[l] = (float)dyda[l];
       c->ytemp[jp] = ymod;
       if (c->Inrel[i] == 1) ave = ave + ymod;
       for (l = 1; l <= ma; l++) {
@@ -1150,7 +1156,7 @@ int *orc_iptr(orc_ctx *c, const char *nm) {
 #define GETSET(X)                                  \
   X(Lmax, int) X(Mmax, int) X(Ncoef, int) X(Numfac, int) X(Lcurves, int) X(Nphpar, int) \
   X(ndata, int) X(ndata_real, int) X(nrows, int) X(ndir, int) X(onlyrel, int) X(ia_prd, int) \
-  X(ia_cl, int) X(n_iter_min_in, int) X(new_conw, int) X(Niter, int) \
+  X(ia_cl, int) X(n_iter_min_in, int) X(new_conw, int) X(Niter, int) X(dyda_f32, int) \
   X(Ochisq, double) X(Chisq, double) X(Alamda, double) X(Alamda_incr, double) \
   X(Alamda_start, double) X(Phi_0, double) X(Scale, double) X(per_start, double) \
   X(per_step_coef, double) X(per_end, double) X(jd_00, double) X(conw, double) X(cl, double) \

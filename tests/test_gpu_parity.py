@@ -363,10 +363,18 @@ def test_eval_at_the_input_limits(tmp_path, n_dense, dense_pts, n_sparse, n_lc):
 #     fp32 (3.1e-6 / 8.0e-6), bf16 (2.6e-3 / 0.67), split bf16 (4.2e-6 / 3.6e-4);
 #   the best row of a 41-period window around the true period: the same row in
 #   every form; its chi2 against the FP64 GPU row (which matches the oracle's
-#   ten fits of that period to 2.5e-12): fp32 and split bf16 2.1e-4 lower
-#   (their 50 LM iterations end at a lower point), bf16 1.5e-5.
-C2 = [("fp32", 1, 1e-5, 1e-4, 5e-4), ("bf16", 2, 1e-2, 1.0, 5e-4), ("bf16x3", 3, 1e-4, 2e-3, 5e-4),
-      ("bf16x6", 4, 1e-5, 1e-4, 5e-4)]
+#   ten fits of that period to 2.5e-12): bf16x6 8.9e-6 and bf16 1.5e-5, held
+#   to 1e-4; fp32 and bf16x3 2.1e-4 lower.  These fits are not converged at
+#   50 iterations: the FP64 best fit still lowers chi2 by about 8.5e-4 per
+#   iteration over its last five (the oracle's own trace: 0.43% from
+#   iteration 45 to 50, 0.68% more by iteration 100), so a form whose
+#   rounding moves the path ends a fraction of an iteration apart; the
+#   oracle itself with its derivatives rounded to FP32 moves the best row by
+#   4.8e-8 only.  fp32 and bf16x3 are held to 5e-4 and to below the FP64
+#   fit's mean chi2 progress per iteration over iterations 46-50, measured
+#   here (5.2e-3 on this row).
+C2 = [("fp32", 1, 1e-5, 1e-4, None), ("bf16", 2, 1e-2, 1.0, 1e-4), ("bf16x3", 3, 1e-4, 2e-3, None),
+      ("bf16x6", 4, 1e-5, 1e-4, 1e-4)]
 
 
 @pytest.fixture(scope="module")
@@ -385,14 +393,28 @@ def c2_scan(tmp_path_factory):
     eng.set_problem(inp.problem, keep=inp)
     rows = eng.scan(inp.freq_start, inp.freq_step, kt - 20, kt + 20, inp.conw_r0)
     eng.close()
-    return path, inp, orc, kt, rows
+    # the best row's ten fits stopped after 45 iterations: chi2 progress per
+    # iteration over the last five of the 50
+    k64 = int(np.argmin(rows["chisq"]))
+    path45 = str(tmp_path_factory.mktemp("c2_45") / "period_search_in")
+    synth.write_workload("big", path45, incl_deg=10.0, stop=45)
+    inp45 = Input(path45)
+    e45 = Engine(0)
+    e45.set_precision(PS_PREC_FP64)
+    e45.set_problem(inp45.problem, keep=inp45)
+    f = inp.freq_start - (kt - 20 + k64 - 1) * inp.freq_step
+    chi45 = e45.fit_batch([f] * 10, list(range(10)), inp.conw_r0)["chisq"].min()
+    e45.close()
+    chi50 = rows["chisq"][k64]
+    progress = (chi45 - chi50) / chi50 / 5
+    return path, inp, orc, kt, rows, progress
 
 
 @pytest.mark.parametrize("name,prec,tol_alpha,tol_beta,tol_chi2", C2, ids=[m[0] for m in C2])
 def test_mixed_forms_at_configuration_2_shape(c2_scan, name, prec, tol_alpha, tol_beta, tol_chi2):
     from periodsearch_amd import Engine
 
-    path, inp, orc, kt, rows64 = c2_scan
+    path, inp, orc, kt, rows64, progress = c2_scan
     nc = orc.iget("Ncoef")
     assert nc == 81
     cgs = []
@@ -416,6 +438,10 @@ def test_mixed_forms_at_configuration_2_shape(c2_scan, name, prec, tol_alpha, to
         assert ea < tol_alpha and eb < tol_beta and _rel(chi[i], c) < 1e-6, (i, ea, eb)
     k64, k = int(np.argmin(rows64["chisq"])), int(np.argmin(rows["chisq"]))
     step_h = rows64["period_h"][k64] ** 2 * inp.freq_step
-    print(f"{name}: best row {k} (FP64 {k64}), chi2 {rows['chisq'][k]:.9g} vs {rows64['chisq'][k64]:.9g}")
+    bar = tol_chi2 if tol_chi2 is not None else min(5e-4, max(1e-4, progress))
+    print(f"{name}: best row {k} (FP64 {k64}), chi2 {rows['chisq'][k]:.9g} vs {rows64['chisq'][k64]:.9g} "
+          f"(rel {_rel(rows['chisq'][k], rows64['chisq'][k64]):.2e}); FP64 progress per iteration at 46-50 "
+          f"{progress:.2e}; bar {bar:.2e}")
+    assert progress > 0
     assert abs(rows["period_h"][k] - rows64["period_h"][k64]) <= step_h
-    assert _rel(rows["chisq"][k], rows64["chisq"][k64]) < tol_chi2
+    assert _rel(rows["chisq"][k], rows64["chisq"][k64]) < bar

@@ -16,6 +16,8 @@
 * How far the reference arithmetic itself moves a 50-iteration fit when its
   inputs move by one ulp: the bar against which the GPU's FP64 fits are held
   (tests/test_gpu_parity.py).
+* The oracle's FP32-derivative switch, which measures what an FP32 Jacobian
+  alone does to the reference arithmetic.
 """
 import os
 
@@ -144,3 +146,24 @@ def test_one_ulp_moves_50_iteration_fits_beyond_1e9(incl_path):
     print("1-ulp time perturbation, chi2 relative change per fit:", " ".join("%.1e" % v for v in spread))
     assert (spread > 1e-9).sum() >= 3
     assert np.median(spread) < 1e-6
+
+
+def test_fp32_derivative_switch(incl_path):
+    """The oracle's dyda_f32 switch (its derivatives rounded to float: what a
+    Jacobian held in FP32 alone does to the reference arithmetic, used to
+    explain the mixed forms' chi2 at the 5,000-point shape): off by default
+    and when cleared; on, alpha and beta move by FP32 rounding, between 1e-12
+    and 1e-6 of their largest entry, while chi2, which uses no derivative,
+    stays bitwise the same."""
+    inp, o = _orc(incl_path)
+    cg = o.initial_cg(inp.freq_start, 3)
+    A0, b0, c0 = o.mrqcof(cg)
+    o.set("dyda_f32", 1)
+    A1, b1, c1 = o.mrqcof(cg)
+    o.set("dyda_f32", 0)
+    A2, b2, c2 = o.mrqcof(cg)
+    assert np.array_equal(A0, A2) and np.array_equal(b0, b2) and c0 == c2
+    ea = np.abs(A1 - A0).max() / np.abs(A0).max()
+    eb = np.abs(b1 - b0).max() / np.abs(b0).max()
+    assert 1e-12 < ea < 1e-6 and 1e-12 < eb < 1e-6, (ea, eb)
+    assert c1 == c0
