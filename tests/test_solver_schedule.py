@@ -1,7 +1,10 @@
 """The register solvers' schedule against the reference's Gauss-Jordan (no GPU).
 
-gauss_solve_block (ps_engine.hip) keeps [A|b] in registers of four waves and
-never moves a row: the reference's row swaps (period_search/gauss_errc.c:47-53)
+Both register solvers of ps_engine.hip -- gauss_solve_wave (one wave, up to
+63 unknowns: lane c holds column c) and gauss_solve_block (four waves, 64-127
+unknowns: lane l holds columns l and l + 64, wave w the rows w + 4r) -- run
+the same schedule, which this file restates.  They keep [A|b] in registers
+and never move a row: the reference's row swaps (period_search/gauss_errc.c:47-53)
 become a logical-row permutation, pivots are found while eliminating, and
 ties are judged by the logical row-major index (the reference's `>=` scan
 keeps the last).  This file restates that schedule in numpy -- physical rows
@@ -10,8 +13,8 @@ a - pr * dum with one rounding per operation as in the kernel (FMA
 contraction off) -- and requires the solution to be bitwise the oracle's
 gauss_errc (itself bitwise the reference's, tests/test_oracle_ref.py) on
 random SPD matrices like LM normal matrices, tie-heavy integer matrices and
-singular ones, for the sizes the kernel takes (64-127 unknowns) and small
-ones.  The GPU kernel is held to the oracle through whole fits
+singular ones, for the sizes the two solvers take (3-63 and 64-127
+unknowns).  The GPU kernels are held to the oracle through whole fits
 (test_gpu_parity.py::test_fits_match_oracle_across_solver_sizes)."""
 import numpy as np
 import pytest
@@ -20,7 +23,7 @@ import oracle
 
 
 def block_schedule(A, b):
-    """The four-wave solver's arithmetic and pivot order.  Returns (code, x)."""
+    """The register solvers' arithmetic and pivot order.  Returns (code, x)."""
     n = len(b)
     M = np.concatenate([A, b[:, None]], axis=1).astype(np.float64)
     perm = np.arange(n)  # logical row -> physical row
@@ -90,7 +93,7 @@ def _spd(rng, n):
     return A
 
 
-@pytest.mark.parametrize("n", [3, 17, 64, 65, 87, 100, 127])
+@pytest.mark.parametrize("n", [3, 17, 40, 63, 64, 65, 87, 100, 127])
 def test_block_schedule_is_bitwise_gauss_errc(n):
     rng = np.random.default_rng(n)
     cases = [(_spd(rng, n), rng.normal(size=n)) for _ in range(3)]
