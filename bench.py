@@ -62,6 +62,7 @@ HOST_CPU_SHARE = 16  # CPU threads the GPU box allots to one GPU
 MODEL_HBM_BYTES_PER_EVAL = {
     ("wu", "fp64"): 469.4e3, ("wu", "fp32"): 242.2e3, ("wu", "bf16"): 242.2e3, ("wu", "bf16x3"): 242.2e3,
     ("wu", "bf16x6"): 242.2e3,
+    ("c2", "fp64"): 4186.9e3, ("c2", "bf16x6"): 2316.6e3,
 }
 PREC_NAMES = ("fp64", "fp32", "bf16", "bf16x3", "bf16x6")
 DTYPE = {"fp64": "f64",
@@ -344,7 +345,8 @@ def main() -> int:
             out["vs_fp64"]["mixed_alone"] = round(out["mixed_fits_per_s"] / r, 3)
 
     # profiled pass of the headline form (outside the timed region): the
-    # model kernel per launch, HIP events captured into the poll graph
+    # model kernel per launch, timed with HIP events between plain launches
+    # (this runtime cannot time events captured into a graph)
     st = None
     if not args.no_profile:
         eng.set_profiling(True)
@@ -426,6 +428,7 @@ def main() -> int:
             "traffic": round(tb * evals_launch) if tb else None,
             "kernel": "model_mixed_kernel" if args.prec != "fp64" else "model_mfma_kernel",
             "ms_per_launch": round(ms_launch, 4), "evals_per_launch": round(evals_launch, 1),
+            "launches": int(st["model_launches"]),
             "flop_per_eval": flop_eval,
             "timing": "HIP events around each model launch, in a profiled pass after the timed regions (plain "
                       "launches: events inside a HIP graph cannot be timed)",
