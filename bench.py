@@ -276,7 +276,7 @@ def main() -> int:
     block = args.block or (2240 if args.workload == "c2" else 4096)
     learned = args.form == "learned"
     n_true = int(round((inp.freq_start - 1.0 / ast["period_h"]) / inp.freq_step)) + 1
-    # learned steps: the whole work unit (c2), or consecutive blocks (wu)
+    # learned steps: the whole work unit (c2), or one block around the true period (wu)
     span = inp.n_periods if args.workload == "c2" else block
     judge_cap = 10 * args.candidates * (2 * args.halo + 1)
 
@@ -294,14 +294,19 @@ def main() -> int:
     N, F = inp.ndata, inp.problem.n_fac
     prop = Proposal.load() if learned else None
 
-    def time_leg(fn, steps, warmup):
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    def time_leg(fn, steps, warmup, name):
         for k in range(warmup):
             fn(k)
         t0 = time.perf_counter()
         work = 0
         for k in range(steps):
             work += fn(warmup + k)
-        return work, time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        log(f"{name}: {work} LM fits in {dt:.2f} s")
+        return work, dt
 
     def plain_step(e):
         def fn(k):
@@ -313,7 +318,9 @@ def main() -> int:
     last = {}
 
     def learned_step(k):
-        n0 = 1 if args.workload == "c2" else 1 + (k * span) % max(1, inp.n_periods - span)
+        # the whole range (c2), or the block of trial periods around the true
+        # period (wu), the same in every step
+        n0 = 1 if args.workload == "c2" else max(1, min(n_true - span // 2, inp.n_periods - span + 1))
         n1 = min(inp.n_periods, n0 + span - 1)
         rows, judged, nf, _ = scan_learned(eng, inp, prop, conw_r, n0, n1, n_candidates=args.candidates,
                                            halo=args.halo, block=block, judge=e64)
@@ -322,17 +329,17 @@ def main() -> int:
 
     out = {}
     if not args.only_headline:
-        fits64, wall64 = time_leg(plain_step(e64), args.steps, args.warmup)
+        fits64, wall64 = time_leg(plain_step(e64), args.steps, args.warmup, "plain fp64 leg")
         out["fp64_fits_per_s"] = round(fits64 / wall64, 2)
         if args.prec != "fp64":
-            fitsm, wallm = time_leg(plain_step(eng), args.steps, args.warmup)
+            fitsm, wallm = time_leg(plain_step(eng), args.steps, args.warmup, f"plain {args.prec} leg")
             out["mixed_fits_per_s"] = round(fitsm / wallm, 2)
     if learned:
-        lm_fits, wall = time_leg(learned_step, args.steps, args.warmup)
+        lm_fits, wall = time_leg(learned_step, args.steps, args.warmup, f"learned {args.prec} leg (headline)")
         periods = args.steps * (last["n1"] - last["n0"] + 1)
         value = lm_fits / wall
     else:
-        lm_fits, wall = time_leg(plain_step(eng), args.steps, args.warmup)
+        lm_fits, wall = time_leg(plain_step(eng), args.steps, args.warmup, f"plain {args.prec} leg (headline)")
         periods = args.steps * P
         value = lm_fits / wall
     out["periods_per_s"] = round(periods / wall, 2)
@@ -353,6 +360,7 @@ def main() -> int:
         (learned_step if learned else plain_step(eng))(0)
         st = eng.kernel_stats()
         eng.set_profiling(False)
+        log("profiled pass done")
 
     def matched_check() -> dict:
         """The best judged row of the headline scan against the plain FP64
@@ -438,13 +446,16 @@ def main() -> int:
         }
     if not args.no_check and (learned or args.prec != "fp64"):
         res["matched_chi2"] = matched_check()
+        log("matched-chi2 check done")
     if not args.no_cpu:
         n_cpu = min(HOST_CPU_SHARE, os.cpu_count() or 1)
         try:
             n_cpu = min(n_cpu, len(os.sched_getaffinity(0)))
         except AttributeError:
             pass
+        log(f"cpu baseline on {n_cpu} cores")
         res["cpu_baseline"] = cpu_baseline(path, conw_r, n_cpu, 1 if args.workload == "c2" else 10)
+        log("cpu baseline done")
         if res["cpu_baseline"]:
             cb = res["cpu_baseline"]["value"]
             res["vs_cpu"] = {"lm_fits_per_s": round(value / cb, 1),
