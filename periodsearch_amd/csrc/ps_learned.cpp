@@ -387,15 +387,17 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
   };
 
   std::vector<float> feat, x, y;
-  std::vector<double> cg;
+  std::vector<double> cg, cg_next;
   std::vector<ps_fit_out> out;
   std::vector<ps_row> brow;
   const double *cgf = p->cg_first;
-  for (int b0 = 0; b0 < N; b0 += opt->block) {
-    const int b1 = std::min(N, b0 + opt->block), nb = b1 - b0;
+  // the starts of the block [b0, b0 + nb): features, the network, and the
+  // reference's start with the proposed pole and l <= 2 shape
+  auto prepare = [&](int b0, int nb, std::vector<double> &cgb) -> int {
     // 1. features and the network
     feat.assign((size_t)nb * (net->nbins + 2), 0.0f);
-    if ((rc = ps_lc_features(e, freq.data() + b0, nb, net->nbins, feat.data()))) break;
+    int r = ps_lc_features(e, freq.data() + b0, nb, net->nbins, feat.data());
+    if (r) return r;
     x.assign((size_t)nb * n_in, 0.0f);
     for (int i = 0; i < nb; i++) {
       float *xi = x.data() + (size_t)i * n_in;
@@ -406,15 +408,14 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
     y.assign((size_t)nb * n_out, 0.0f);
     for (int i0 = 0; i0 < nb; i0 += net->max_batch) {
       const int nn = std::min(net->max_batch, nb - i0);
-      if ((rc = ps_mlp_forward(net->mlp, x.data() + (size_t)i0 * n_in, nn, y.data() + (size_t)i0 * n_out))) {
+      if ((r = ps_mlp_forward(net->mlp, x.data() + (size_t)i0 * n_in, nn, y.data() + (size_t)i0 * n_out))) {
         psamd_engine_error(e, std::string("network: ") + ps_mlp_last_error(net->mlp));
-        break;
+        return r;
       }
     }
-    if (rc) break;
     // 2. starts: the reference's start (period_search_BOINC.cpp:819-844)
     //    with the proposed pole and l <= 2 coefficients (c00, the size, kept)
-    cg.assign((size_t)nb * per * ma, 0.0);
+    cgb.assign((size_t)nb * per * ma, 0.0);
     for (int i = 0; i < nb; i++) {
       double yv[12];
       for (int k = 0; k < n_out; k++)
@@ -423,7 +424,7 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
       const double beta = std::asin(std::min(1.0, std::max(-1.0, yv[2] / vn)));
       const double lam = 0.5 * std::atan2(yv[1], yv[0]);
       for (int k = 0; k < per; k++) {
-        double *a = cg.data() + ((size_t)i * per + k) * ma;
+        double *a = cgb.data() + ((size_t)i * per + k) * ma;
         for (int c = 0; c < nc; c++) a[c] = cgf[c];
         for (int w = 1; w < 9; w++) a[idx[w]] = (k == 1 && m1[w]) ? -yv[3 + w] : yv[3 + w];
         a[nc] = kPi / 2 - beta;
@@ -437,6 +438,11 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
         a[nc + 7] = 1;
       }
     }
+    return PS_OK;
+  };
+  rc = prepare(0, std::min(N, opt->block), cg);
+  for (int b0 = 0; b0 < N && !rc; b0 += opt->block) {
+    const int b1 = std::min(N, b0 + opt->block), nb = b1 - b0;
     // 3. screening fits (the previous block's judge runs meanwhile)
     out.assign((size_t)nb * per, ps_fit_out{});
     rc = ps_fit_batch_from(e, cg.data(), nb * per, conw_r, out.data(), nullptr);
@@ -450,11 +456,15 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
       rows[b0 + i] = brow[i];
       if (screened) screened[b0 + i] = brow[i];
     }
+    // the next block's starts before this block's judge starts, so that the
+    // features and the network never wait behind the judge's kernels
+    if (b1 < N && (rc = prepare(b1, std::min(N, b1 + opt->block) - b1, cg_next))) break;
     // 4. candidates of this block go to the judge
     std::vector<int> cand = pick_candidates(brow.data(), nb, opt->n_candidates, opt->halo);
     running.pos.clear();
     for (int i : cand) running.pos.push_back(b0 + i);
     start_judge(running);
+    cg.swap(cg_next);
   }
   if (rc) {
     if (worker.joinable()) worker.join();
