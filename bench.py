@@ -62,7 +62,7 @@ HOST_CPU_SHARE = 16  # CPU threads the GPU box allots to one GPU
 MODEL_HBM_BYTES_PER_EVAL = {
     ("wu", "fp64"): 469.4e3, ("wu", "fp32"): 242.2e3, ("wu", "bf16"): 242.2e3, ("wu", "bf16x3"): 242.2e3,
     ("wu", "bf16x6"): 242.2e3,
-    ("c2", "fp64"): 4186.9e3, ("c2", "bf16x6"): 2316.6e3,
+    ("c2", "fp64"): 4186.9e3, ("c2", "bf16x6"): 2316.6e3, ("c2", "bf16"): 2270.8e3,
 }
 PREC_NAMES = ("fp64", "fp32", "bf16", "bf16x3", "bf16x6")
 DTYPE = {"fp64": "f64",
@@ -228,7 +228,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=("c2", "wu", "batch64"), default="c2")
     ap.add_argument("--periods", type=int, default=0,
                     help="trial periods per step of the plain FP64 and mixed legs (0: 256 for c2, 2048 for wu)")
-    ap.add_argument("--prec", choices=PREC_NAMES, default="bf16x6")
+    ap.add_argument("--prec", choices=PREC_NAMES, default="bf16",
+                    help="precision of the mixed leg and of the learned form's screening fits (the judge is FP64)")
     ap.add_argument("--form", choices=("plain", "learned"), default="learned")
     ap.add_argument("--candidates", type=int, default=8, help="learned form: periods judged per block")
     ap.add_argument("--halo", type=int, default=1, help="learned form: neighbours judged on either side")
