@@ -97,7 +97,8 @@ struct DevProblem {
   const double *dsph;     // [F][ncp], zero-padded columns
   // mixed-precision tables: normals, and Dsph transposed [16*nb][fpad] (zero
   // beyond ncoef and beyond F) in FP32 and in bf16
-  const float4 *nor4f;
+  const float *nor3f;      // [fpad][3] (nx, ny, nz), zero past the last facet: a wave's eight
+                           // normals are 24 consecutive floats, fetched by wide scalar loads
   const float *dsphT32;
   const unsigned short *dsphT16, *dsphT16lo;  // bf16 of Dsph, and of Dsph - hi (split forms)
   const unsigned short *dsphT16lo2;           // bf16 of Dsph - hi - lo (three-part split form)
@@ -1758,7 +1759,7 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   v4f acc[NB];
 #pragma unroll
   for (int cb = 0; cb < NB; cb++) acc[cb] = v4f{0, 0, 0, 0};
-  const float4 *__restrict__ n4 = P.nor4f;
+  const float *__restrict__ n3 = P.nor3f;
   const T *__restrict__ dsT = BF ? reinterpret_cast<const T *>(P.dsphT16) : reinterpret_cast<const T *>(P.dsphT32);
   const T *__restrict__ dsTl = reinterpret_cast<const T *>(P.dsphT16lo);
   const T *__restrict__ dsTl2 = reinterpret_cast<const T *>(P.dsphT16lo2);
@@ -1794,22 +1795,22 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
     if (f0 + kFacetChunk < F) load_chunk(f0 + kFacetChunk);
     // facet stage (FP32).  The eight facets of a wave are wave-uniform: their
     // normals (scalar loads) and areas are fetched before the first use.
+    // fb + 8 <= fpad (fpad is a multiple of the chunk), so the 24 floats of
+    // the normals are inside the zero-padded table and need no clamp.
     const int fb = __builtin_amdgcn_readfirstlane(f0 + 8 * sub);
-    float4 nvs[8];
+    float nv3[24];
     float afs[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int f = min(fb + i, F - 1);
-      nvs[i] = n4[f];
-      afs[i] = areaf[f];
-    }
+    for (int k = 0; k < 24; k++) nv3[k] = n3[3 * fb + k];
+#pragma unroll
+    for (int i = 0; i < 8; i++) afs[i] = areaf[min(fb + i, F - 1)];
     float wv[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int f = fb + i;
       float w = 0.0f;
       if (f < F) {
-        const float4 nv = nvs[i];
+        const float3 nv = make_float3(nv3[3 * i], nv3[3 * i + 1], nv3[3 * i + 2]);
         const float mu = e_0 * nv.x + e_1 * nv.y + e_2 * nv.z;
         const float mu0 = s_0 * nv.x + s_1 * nv.y + s_2 * nv.z;
         if (mu > tiny && mu0 > tiny) {
@@ -2832,7 +2833,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   feat[1] = (float)(sb / N);
   std::vector<double> nx(F), ny(F), nz(F), dsph((size_t)F * ncp, 0.0);
   std::vector<double4> nor4(F);
-  std::vector<float4> nor4f(F);
+  std::vector<float> nor3f;
   const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk, ncT = 16 * nb;
   std::vector<float> dsphT32((size_t)ncT * fpad, 0.0f);
   std::vector<unsigned short> dsphT16((size_t)ncT * fpad, 0), dsphT16lo((size_t)ncT * fpad, 0),
@@ -2866,7 +2867,9 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
       dsphT16lo2[(size_t)c * fpad + f] = to_bf16((vf - hf) - lf);
     }
     nor4[f] = make_double4(nx[f], ny[f], nz[f], 0.0);
-    nor4f[f] = make_float4((float)nx[f], (float)ny[f], (float)nz[f], 0.0f);
+    nor3f.push_back((float)nx[f]);
+    nor3f.push_back((float)ny[f]);
+    nor3f.push_back((float)nz[f]);
   }
   const int L1 = p->lmax + 1, M1 = p->mmax + 1;
   std::vector<double> cpleg((size_t)ngrp * F), cfc((size_t)M1 * F), cfs((size_t)M1 * F);
@@ -2903,7 +2906,8 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   UP(dsph.data(), dsph.size(), dsph);
   UP(dsphP.data(), dsphP.size(), dsphP);
   UP(nor4.data(), F, nor4);
-  UP(nor4f.data(), F, nor4f);
+  nor3f.resize((size_t)3 * fpad, 0.0f);
+  UP(nor3f.data(), nor3f.size(), nor3f);
   UP(dsphT32.data(), dsphT32.size(), dsphT32);
   UP(dsphT16.data(), dsphT16.size(), dsphT16);
   UP(dsphT16lo.data(), dsphT16lo.size(), dsphT16lo);
