@@ -88,6 +88,7 @@ __device__ __forceinline__ void static_for(Fn &&f) {
 struct DevProblem {
   // points, SoA, real points only
   const double *tim, *y, *sig, *w, *ex, *ey, *ez, *sx, *sy, *sz;
+  const double *swt;    // [n_points] w / sig^2 (mixed-precision normal equations)
   const float *ynorm;   // [n_points] brightness / light-curve mean - 1 (proposal features)
   const int *lc_start;  // [n_lc + 2]: real light curves then the conv "light curve"
   const int *lc_inrel;  // [n_lc + 1]
@@ -742,6 +743,289 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
   }
   if (tid < mfit) beta[tid] = bacc;
   if (tid == kThreads - 1) *chi_out = chi;
+}
+
+// Mixed-precision normal equations with the points split over the waves
+// (mrqcof.c:42-139).  Lane (c, q) = (lane % 16, lane / 16) owns free
+// parameter 16*b + c of every column block b and the points 4q..4q+3 of a
+// 16-point group, whose FP32 derivative rows it reads straight from the
+// Jacobian as one 16-byte load per block: no staging through LDS.  Step s of
+// a group pairs the four lanes' point 4q+s, so one v_mfma_f32_16x16x4f32 per
+// (lower-triangle tile, s) sums four distinct points and the four steps sum
+// all sixteen.  Every wave accumulates all NB(NB+1)/2 tiles over its own
+// groups (group g goes to wave g % 4 within each light curve); the four
+// waves' tiles are added in wave order at the end.  The loads of a wave's
+// next group are issued before its current group is computed.
+// Per-point values are computed in FP64 once per point, by lane
+// 16q + s for point 4q + s: the renormalisation factor coef = sig*np/ave,
+// the renormalised model value coef*ymod, the residual dy and the weight
+// w/sig^2; chi2 sums dy^2 w/sig^2 there.  They reach the other lanes of the
+// row as FP32 by a DPP row broadcast.
+// A relative light curve takes two passes: its column sums ave and dave
+// (FP32 within a group of 16 points, FP64 across groups), then the rows
+// renormalised in FP32 as coef*dyda - (coef*ymod)*(dave/ave), with the
+// size-coefficient column zeroed.  A group that straddles two light curves
+// is read once for each, with the other curve's points masked.
+// beta is FP64 across groups: each lane sums its four points' Jn*w*dy/sig^2
+// per column in FP32, adds that to its FP64 sum, and the lanes of a column
+// and the waves are added in a fixed order at the end, so the result does
+// not depend on timing.
+// LDS (doubles): [4][16*NB+1] column sums, [4][16*NB] beta, [4] chi2, then
+// 4 tiles x 4 waves x 256 FP32 partial tiles.
+__host__ __device__ constexpr int neq_ks_doubles(int nb) { return 4 * (16 * nb + 1) + 4 * 16 * nb + 4 + 2048; }
+
+// v broadcast from lane i of its row of 16 lanes (DPP row_newbcast)
+__device__ __forceinline__ float row_bcast(float v, int i) {
+  const int x = __builtin_bit_cast(int, v);
+  int r;
+  switch (i) {
+    case 0: r = __builtin_amdgcn_update_dpp(0, x, 0x150, 0xF, 0xF, false); break;
+    case 1: r = __builtin_amdgcn_update_dpp(0, x, 0x151, 0xF, 0xF, false); break;
+    case 2: r = __builtin_amdgcn_update_dpp(0, x, 0x152, 0xF, 0xF, false); break;
+    default: r = __builtin_amdgcn_update_dpp(0, x, 0x153, 0xF, 0xF, false); break;
+  }
+  return __builtin_bit_cast(float, r);
+}
+
+template <int NB>
+__device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const double *J, double conw_r,
+                                                    double *ks, double *dave, double *drat, const int *fidx,
+                                                    double *beta, double *chi_out, double *alpha) {
+#pragma clang fp contract(off)
+  constexpr int NT = NB * (NB + 1) / 2, NC = 16 * NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c16 = lane & 15, kq = lane >> 4, i4 = c16 & 3;
+  const int ld = P.npad, mfit = P.mfit, ma = P.ma, npts = P.n_points;
+  const float *Jf = reinterpret_cast<const float *>(J);
+  const double *ym = J + (size_t)ma * ld;  // FP64 model row
+  double *csum = ks, *bsum = ks + 4 * (NC + 1), *csq = bsum + 4 * NC;
+  float *tred = reinterpret_cast<float *>(csq + 4);
+  const double sconv = 1 / conw_r, swconv = 1 / (sconv * sconv);  // convexity rows: sigma, weight/sigma^2
+  // this lane's column of each block: offset of its Jacobian row (free
+  // parameter 16*b + c; padding columns read row 0 and land in discarded
+  // rows and columns of alpha and beta), and whether it is the size
+  // coefficient (zeroed in relative light curves)
+  int roff[NB];
+  bool l0[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int jf = 16 * b + c16;
+    const int l = jf < mfit ? fidx[jf] : 0;
+    roff[b] = l * ld + 4 * kq;
+    l0[b] = l == 0;
+  }
+  v4f acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = v4f{0, 0, 0, 0};
+  double bet[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) bet[b] = 0;
+  double chi = 0;
+  // one group's loads: the rows of this lane's four points, and the model
+  // value and data of its own point 4q + c%4 (the convexity rows and the
+  // padding past them read the last real point's data, replaced at use)
+  struct Grp {
+    v4f raw[NB];
+    double ymq, sgq, swq, yoq;
+  };
+  auto load = [&](int g, Grp &G, bool pts) __attribute__((always_inline)) {
+#pragma unroll
+    for (int b = 0; b < NB; b++) G.raw[b] = *reinterpret_cast<const v4f *>(Jf + roff[b] + 16 * g);
+    const int q = 16 * g + 4 * kq + i4, qc = min(q, npts - 1);
+    G.ymq = ym[q];
+    if (pts) {
+      G.sgq = P.sig[qc];
+      G.swq = P.swt[qc];
+      G.yoq = P.y[qc];
+    }
+  };
+  for (int lc = 0; lc <= P.n_lc; lc++) {
+    const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
+    const bool inrel = P.lc_inrel[lc] != 0;
+    const int g0 = q0 >> 4, g1 = (q1 + 15) >> 4;
+    double rave = 0;  // np / ave
+    float dr[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) dr[b] = 0;
+    if (inrel) {
+      // pass 1: ave = sum ymod, dave = sum dyda over the light curve
+      double sy = 0, sd[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++) sd[b] = 0;
+      Grp cur, nxt;
+      int g = g0 + wave;
+      if (g < g1) load(g, nxt, false);
+      for (; g < g1; g += kWaves) {
+        cur = nxt;
+        if (g + kWaves < g1) load(g + kWaves, nxt, false);
+        const int pb = 16 * g + 4 * kq;
+        {
+          const int q = pb + i4;
+          sy = sy + ((c16 < 4 && q >= q0 && q < q1) ? cur.ymq : 0.0);
+        }
+        if (16 * g >= q0 && 16 * g + 16 <= q1) {  // wave-uniform: every point of the group in the curve
+#pragma unroll
+          for (int b = 0; b < NB; b++)
+            sd[b] = sd[b] + (double)((cur.raw[b][0] + cur.raw[b][1]) + (cur.raw[b][2] + cur.raw[b][3]));
+        } else {
+          bool lv[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) lv[i] = pb + i >= q0 && pb + i < q1;
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            const float s = ((lv[0] ? cur.raw[b][0] : 0.0f) + (lv[1] ? cur.raw[b][1] : 0.0f)) +
+                            ((lv[2] ? cur.raw[b][2] : 0.0f) + (lv[3] ? cur.raw[b][3] : 0.0f));
+            sd[b] = sd[b] + (double)s;
+          }
+        }
+      }
+      // lanes of one column, then waves, in a fixed order
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        sd[b] = sd[b] + __shfl_xor(sd[b], 16, 64);
+        sd[b] = sd[b] + __shfl_xor(sd[b], 32, 64);
+      }
+      // the model-value sum: lanes c < 4 of each row hold its points
+      sy = sy + __shfl_xor(sy, 1, 64);
+      sy = sy + __shfl_xor(sy, 2, 64);
+      sy = sy + __shfl_xor(sy, 16, 64);
+      sy = sy + __shfl_xor(sy, 32, 64);
+      if (kq == 0) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) csum[wave * (NC + 1) + 16 * b + c16] = sd[b];
+        if (c16 == 0) csum[wave * (NC + 1) + NC] = sy;
+      }
+      __syncthreads();
+      if (tid <= NC) {
+        double s = csum[tid];
+        for (int w = 1; w < kWaves; w++) s = s + csum[w * (NC + 1) + tid];
+        dave[tid] = s;
+      }
+      __syncthreads();
+      const double ave = dave[NC];
+      rave = np / ave;
+      if (tid < NC) drat[tid] = dave[tid] / ave;
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB; b++) dr[b] = (float)drat[16 * b + c16];
+    }
+    // pass 2: renormalised rows, normal equations, beta and chi2
+    Grp cur, nxt;
+    int g = g0 + wave;
+    if (g < g1) load(g, nxt, true);
+    for (; g < g1; g += kWaves) {
+      cur = nxt;
+      if (g + kWaves < g1) load(g + kWaves, nxt, true);
+      const int pb = 16 * g + 4 * kq;
+      const bool full = 16 * g >= q0 && 16 * g + 16 <= q1;  // wave-uniform
+      // this lane's own point (FP64): renormalisation factor and model
+      // value, residual, weight; chi2
+      float coef_f, cy_f, sw_f, swdy_f;
+      {
+        const int q = pb + i4;
+        const bool live = q >= q0 && q < q1, real = q < npts;
+        const double sg = real ? cur.sgq : sconv, sw = live ? (real ? cur.swq : swconv) : 0.0;
+        const double yo = real ? cur.yoq : 0.0;
+        const double coef = inrel ? sg * rave : 1.0;
+        const double ymr = live ? cur.ymq : 0.0;
+        const double cy = inrel ? coef * ymr : ymr;
+        const double dy = live ? yo - cy : 0.0;
+        if (c16 < 4) chi = chi + dy * dy * sw;
+        coef_f = (float)coef;
+        cy_f = (float)cy;
+        sw_f = (float)sw;
+        swdy_f = (float)(sw * dy);
+      }
+      float bp[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++) bp[b] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float cf = row_bcast(coef_f, i), cyi = row_bcast(cy_f, i), swi = row_bcast(sw_f, i),
+                    swdyi = row_bcast(swdy_f, i);
+        const bool live = full || (pb + i >= q0 && pb + i < q1);
+        float av[NB], bv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          float jn = cur.raw[b][i];
+          if (inrel) jn = l0[b] ? 0.0f : cf * jn - cyi * dr[b];
+          if (!full) jn = live ? jn : 0.0f;
+          bp[b] = bp[b] + jn * swdyi;
+          bv[b] = jn;
+          av[b] = jn * swi;
+        }
+        static_for<0, NB>([&](auto jbc) __attribute__((always_inline)) {
+          constexpr int jb = decltype(jbc)::value;
+          static_for<0, jb + 1>([&](auto kbc) __attribute__((always_inline)) {
+            constexpr int kb = decltype(kbc)::value;
+            constexpr int t = jb * (jb + 1) / 2 + kb;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[jb], bv[kb], acc[t], 0, 0, 0);
+          });
+        });
+      }
+#pragma unroll
+      for (int b = 0; b < NB; b++) bet[b] = bet[b] + (double)bp[b];
+    }
+  }
+  // beta: lanes of a column, then the waves in order
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    bet[b] = bet[b] + __shfl_xor(bet[b], 16, 64);
+    bet[b] = bet[b] + __shfl_xor(bet[b], 32, 64);
+    if (kq == 0) bsum[wave * NC + 16 * b + c16] = bet[b];
+  }
+  // chi2: lanes c < 4 of each row hold its points' terms
+  chi = chi + __shfl_xor(chi, 1, 64);
+  chi = chi + __shfl_xor(chi, 2, 64);
+  chi = chi + __shfl_xor(chi, 16, 64);
+  chi = chi + __shfl_xor(chi, 32, 64);
+  if (lane == 0) csq[wave] = chi;
+  __syncthreads();
+  if (tid < mfit) {
+    double s = bsum[tid];
+    for (int w = 1; w < kWaves; w++) s = s + bsum[w * NC + tid];
+    beta[tid] = s;
+  }
+  if (tid == 0) *chi_out = ((csq[0] + csq[1]) + csq[2]) + csq[3];
+  // alpha: four tiles at a time through LDS, each tile summed over the
+  // waves in order by one wave; j >= k entries stored and mirrored.
+  // FP32 D of 16x16x4: D[4(l/16) + r][l%16]
+  static_for<0, (NT + 3) / 4>([&](auto rc) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    static_for<0, 4>([&](auto uc) __attribute__((always_inline)) {
+      constexpr int u = decltype(uc)::value, t = 4 * R + u;
+      if constexpr (t < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) tred[((u * kWaves + wave) * 4 + r) * 64 + lane] = acc[t][r];
+      }
+    });
+    __syncthreads();
+    const int t = 4 * R + wave;
+    if (t < NT) {
+      int jb = 0;
+      while ((jb + 1) * (jb + 2) / 2 <= t) jb++;
+      const int kb = t - jb * (jb + 1) / 2;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double s = tred[((wave * kWaves + 0) * 4 + r) * 64 + lane];
+        for (int w = 1; w < kWaves; w++) s = s + (double)tred[((wave * kWaves + w) * 4 + r) * 64 + lane];
+        const int j = 16 * jb + 4 * kq + r, k = 16 * kb + c16;
+        if (j < mfit && k < mfit && j >= k) {
+          alpha[j * mfit + k] = s;
+          alpha[k * mfit + j] = s;
+        }
+      }
+    }
+    __syncthreads();
+  });
+}
+
+// Scratch at the start of the normal-equation kernel's LDS: the convexity
+// rows' partial sums, then the staged rows (ks = 0) or the K-split
+// reductions (ks = column blocks).
+__host__ __device__ inline int neq_scratch_doubles(int mfit, bool mix, int ks) {
+  const int d = neq_jn_doubles(mfit, mix);
+  return ks > 0 && neq_ks_doubles(ks) > d ? neq_ks_doubles(ks) : d;
 }
 
 // Fit initialisation (period_search_BOINC.cpp:819-844): shape from the
@@ -1947,8 +2231,11 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
 // the trial parameters, then accept/reject and the driver bookkeeping.
 // LDS: staged rows (neq_jn_doubles), aux [3*kNeqChunk], dave [ma+1], x [ma],
 // scalars [4], dave/ave [ma].
-template <int NT, bool MIX, bool MULTI>
-__global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+#ifndef PS_NEQ_KS_WAVES
+#define PS_NEQ_KS_WAVES 2
+#endif
+template <int NT, bool MIX, bool MULTI, int KS = 0>
+__global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
   SlotState &S = W.st[slot];
@@ -1958,7 +2245,7 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevP
   const int mfit = P.mfit, ma = P.ma, nc = P.ncoef;
   const SlotMem M = slot_mem(ma, mfit, P.npad);
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
-  double *wst = sm, *dave = wst + neq_jn_doubles(mfit, MIX) + 3 * kNeqChunk;
+  double *wst = sm, *dave = wst + neq_scratch_doubles(mfit, MIX, KS) + 3 * kNeqChunk;
   double *x = dave + kMaxMa + 1, *scal = x + kMaxMa;  // scal[0] = chi2, [1..3] = chck
   double *drat = scal + 4;                            // [ma] dave/ave (mixed forms)
   int *fidx = reinterpret_cast<int *>(drat + kMaxMa); // [mfit] free parameter -> index into a[]
@@ -1972,8 +2259,12 @@ __global__ void __launch_bounds__(kThreads) neq_kernel(DevProblem P0, const DevP
     conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1, wst);
     __threadfence_block();
     __syncthreads();
-    normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx,
-                                   btr, scal, atr);
+    if constexpr (KS > 0)
+      normal_equations_ks<KS>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx, btr,
+                              scal, atr);
+    else
+      normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx,
+                                     btr, scal, atr);
     __threadfence_block();
     __syncthreads();
   }
@@ -2238,8 +2529,8 @@ size_t step_lds(int mfit, int nfac) {  // mfit = 0: a register solver (at most 1
 size_t model_lds(int nfac) {
   return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
 }
-size_t neq_lds(int mfit, bool mix) {
-  return ((size_t)neq_jn_doubles(mfit, mix) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8 + (kMaxMfit + 1) / 2) *
+size_t neq_lds(int mfit, bool mix, int ks) {
+  return ((size_t)neq_scratch_doubles(mfit, mix, ks) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8 + (kMaxMfit + 1) / 2) *
          sizeof(double);
 }
 
@@ -2264,8 +2555,26 @@ StepFn pick_step(int mfit) {
 }
 
 using NeqFn = void (*)(DevProblem, const DevProblem *, DevWork);
+// Column blocks of the K-split normal equations for mfit free parameters
+// (mixed forms), or 0: the staged kernel.
+inline int neq_ks_blocks(int mfit, bool mix, bool staged) {
+  const int nb = (mfit + 15) / 16;
+  return mix && !staged && nb <= 6 ? nb : 0;
+}
+
 template <bool MIX, bool MULTI>
-NeqFn pick_neq(int mfit) {
+NeqFn pick_neq(int mfit, int ks) {
+  if constexpr (MIX) {
+    switch (ks) {
+      case 1: return neq_kernel<1, true, MULTI, 1>;
+      case 2: return neq_kernel<1, true, MULTI, 2>;
+      case 3: return neq_kernel<1, true, MULTI, 3>;
+      case 4: return neq_kernel<1, true, MULTI, 4>;
+      case 5: return neq_kernel<1, true, MULTI, 5>;
+      case 6: return neq_kernel<1, true, MULTI, 6>;
+      default: break;
+    }
+  }
   const int nb = (mfit + 15) / 16, ntile = nb * (nb + 1) / 2, nt = (ntile + 3) / 4;
   if (nt <= 1) return neq_kernel<1, MIX, MULTI>;
   if (nt <= 3) return neq_kernel<3, MIX, MULTI>;
@@ -2337,6 +2646,7 @@ struct ps_engine {
   NeqFn neq = nullptr;
   void (*step)(DevProblem, const DevProblem *, DevWork) = nullptr;
   bool use_mfma = true;   // FP64: matrix-core contraction (default) or the VALU kernel
+  bool neq_staged = false;  // mixed forms: the staged normal-equation kernel instead of the K-split one (PSAMD_NEQ=staged)
   // slot memory
   SlotState *st = nullptr;
   double *mem = nullptr;
@@ -2420,7 +2730,8 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   const bool wave_solve = mfit <= kWaveSolveMax, reg_solve = mfit <= kBlockSolveMax;
   ks.lds_step = step_lds(reg_solve ? 0 : mfit, F);
   ks.lds_model = mixed ? mixed_layout(F, nb, mode).total : (ks.use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
-  ks.lds_neq = neq_lds(mfit, mixed);
+  const int neq_blocks = neq_ks_blocks(mfit, mixed, e->neq_staged);
+  ks.lds_neq = neq_lds(mfit, mixed, neq_blocks);
   const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
   if (ks.lds_step > lds_max || ks.lds_neq > lds_max || ks.lds_model > lds_max) {
     e->err = "problem needs more LDS than a CU has";
@@ -2428,8 +2739,8 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   }
   ks.step = multi ? pick_step<true>(mfit) : pick_step<false>(mfit);
   ks.step_threads = wave_solve ? 64 : kThreads;
-  ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit) : pick_neq<true, false>(mfit))
-                 : (multi ? pick_neq<false, true>(mfit) : pick_neq<false, false>(mfit));
+  ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit, neq_blocks) : pick_neq<true, false>(mfit, neq_blocks))
+                 : (multi ? pick_neq<false, true>(mfit, 0) : pick_neq<false, false>(mfit, 0));
   if (!ks.neq) {
     e->err = "too many free parameters";
     return PS_ERR_LIMIT;
@@ -2890,6 +3201,11 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   UP(p->brightness, N, y);
   UP(p->sig, N, sig);
   UP(p->weight, N, w);
+  {
+    std::vector<double> swt(N);
+    for (int q = 0; q < N; q++) swt[q] = (1 / (p->sig[q] * p->sig[q])) * p->weight[q];
+    UP(swt.data(), N, swt);
+  }
   UP(ex.data(), N, ex);
   UP(ey.data(), N, ey);
   UP(ez.data(), N, ez);
@@ -3016,6 +3332,7 @@ int ps_create(int device, ps_engine **out) {
     e->scratch_dev = per_device[device];
   }
   if (const char *gr = getenv("PSAMD_GRAPH")) e->use_graph = atoi(gr) != 0;
+  if (const char *nq = getenv("PSAMD_NEQ")) e->neq_staged = strcmp(nq, "staged") == 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
     e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);

@@ -373,8 +373,13 @@ def test_eval_at_the_input_limits(tmp_path, n_dense, dense_pts, n_sparse, n_lc):
 #   4.8e-8 only.  fp32 and bf16x3 are held to 5e-4 and to below the FP64
 #   fit's mean chi2 progress per iteration over iterations 46-50, measured
 #   here (5.2e-3 on this row).
-C2 = [("fp32", 1, 1e-5, 1e-4, None), ("bf16", 2, 1e-2, 1.0, 1e-4), ("bf16x3", 3, 1e-4, 2e-3, None),
-      ("bf16x6", 4, 1e-5, 1e-4, 1e-4)]
+# (form, precision, alpha bar, beta bar, best-row chi2 bar or None).  None:
+# the best row's chi2 after the 50 iterations is held to what one LM
+# iteration of the FP64 fit still changes there (at least 1e-4, at most
+# 5e-4): these fits are not converged, and any change of rounding moves the
+# 50th iterate along the descent by a fraction of an iteration.
+C2 = [("fp32", 1, 1e-5, 1e-4, None), ("bf16", 2, 1e-2, 1.0, None), ("bf16x3", 3, 1e-4, 2e-3, None),
+      ("bf16x6", 4, 1e-5, 1e-4, None)]
 
 
 @pytest.fixture(scope="module")
