@@ -1971,8 +1971,8 @@ __host__ __device__ inline MixedLds mixed_layout(int nfac, int nb, int mode) {
   o += kMaxMa * sizeof(double);
   L.blm = o;
   o += 32 * sizeof(double);
-  L.area = o;
-  o = align16(o + (size_t)nfac * sizeof(float));
+  L.area = o;  // zero-padded to whole facet chunks
+  o = align16(o + (size_t)((nfac + kFacetChunk - 1) / kFacetChunk * kFacetChunk) * sizeof(float));
   L.geom = o;
   o = align16(o + kTilePts * sizeof(PointGeom));
   L.wt = o;
@@ -1980,7 +1980,7 @@ __host__ __device__ inline MixedLds mixed_layout(int nfac, int nb, int mode) {
   L.dch = o;
   o = align16(o + parts * 16 * nb * ws * es);
   L.red = L.wt;
-  const size_t red = (size_t)kWaves * kTilePts * 9 * sizeof(float);
+  const size_t red = (size_t)kWaves * kTilePts * 8 * sizeof(float);
   if (o - L.wt < red) o = L.wt + red;
   L.total = o;
   return L;
@@ -2023,7 +2023,7 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   float *red = reinterpret_cast<float *>(smb + L.red);
 
   for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
-  for (int f = tid; f < F; f += kThreads) areaf[f] = (float)mem[M.area + f];
+  for (int f = tid; f < P.fpad; f += kThreads) areaf[f] = f < F ? (float)mem[M.area + f] : 0.0f;
   if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
   __syncthreads();
   const int p0 = tile * kTilePts;
@@ -2039,7 +2039,7 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   const float s_0 = (float)gst[pl].e0[0], s_1 = (float)gst[pl].e0[1], s_2 = (float)gst[pl].e0[2];
   const double cld = exp(x[P.ma - 2]);
   const float cl = (float)cld, cls = (float)x[P.ma - 1], tiny = (float)kTiny;
-  float br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
+  float v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
   v4f acc[NB];
 #pragma unroll
   for (int cb = 0; cb < NB; cb++) acc[cb] = v4f{0, 0, 0, 0};
@@ -2083,38 +2083,41 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
     // the normals are inside the zero-padded table and need no clamp.
     const int fb = __builtin_amdgcn_readfirstlane(f0 + 8 * sub);
     float nv3[24];
-    float afs[8];
 #pragma unroll
     for (int k = 0; k < 24; k++) nv3[k] = n3[3 * fb + k];
-#pragma unroll
-    for (int i = 0; i < 8; i++) afs[i] = areaf[min(fb + i, F - 1)];
+    // areas of the eight facets (zero past the last facet, whose zero normals
+    // are never visible)
+    const float4 a03 = *reinterpret_cast<const float4 *>(areaf + fb);
+    const float4 a47 = *reinterpret_cast<const float4 *>(areaf + fb + 4);
+    const float afs[8] = {a03.x, a03.y, a03.z, a03.w, a47.x, a47.y, a47.z, a47.w};
     float wv[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const int f = fb + i;
       float w = 0.0f;
-      if (f < F) {
+      {
         const float3 nv = make_float3(nv3[3 * i], nv3[3 * i + 1], nv3[3 * i + 2]);
         const float mu = e_0 * nv.x + e_1 * nv.y + e_2 * nv.z;
         const float mu0 = s_0 * nv.x + s_1 * nv.y + s_2 * nv.z;
         if (mu > tiny && mu0 > tiny) {
+          // w = Area mu mu0 (cl + cls/(mu+mu0)) and its derivatives
+          // dw/dmu = Area mu0 (cl + cls mu0/(mu+mu0)^2), dw/dmu0 likewise;
+          // the brightness sum is cl*scl + cls*scls, formed after the loop
           const float af = afs[i];
           const float rd = __builtin_amdgcn_rcpf(mu + mu0);
-          const float mm = mu * mu0;
-          w = af * (mm * (cl + cls * rd));
-          br += w;
-          const float q0 = mu0 * rd, q1 = mu * rd;
-          const float ad = af * (cls * q0 * q0 + cl * mu0);
-          const float ad0 = af * (cls * q1 * q1 + cl * mu);
+          const float amu = af * mu, amu0 = af * mu0;
+          const float am = amu * mu0;
+          w = am * (cl + cls * rd);
+          scl += am;
+          scls += am * rd;
+          const float k = cls * (rd * rd);
+          const float ad = amu0 * (cl + k * mu0);
+          const float ad0 = amu * (cl + k * mu);
           v0 += ad * nv.x;
           v1 += ad * nv.y;
           v2 += ad * nv.z;
           u0 += ad0 * nv.x;
           u1 += ad0 * nv.y;
           u2 += ad0 * nv.z;
-          const float am = af * mm;
-          scl += am;
-          scls += am * rd;
         }
       }
       wv[i] = w;
@@ -2183,16 +2186,15 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
     __syncthreads();
   }
   {
-    float *rp = red + ((size_t)sub * kTilePts + pl) * 9;
-    rp[0] = br;
-    rp[1] = v0;
-    rp[2] = v1;
-    rp[3] = v2;
-    rp[4] = u0;
-    rp[5] = u1;
-    rp[6] = u2;
-    rp[7] = scl;
-    rp[8] = scls;
+    float *rp = red + ((size_t)sub * kTilePts + pl) * 8;
+    rp[0] = v0;
+    rp[1] = v1;
+    rp[2] = v2;
+    rp[3] = u0;
+    rp[4] = u1;
+    rp[5] = u2;
+    rp[6] = scl;
+    rp[7] = scls;
   }
   __syncthreads();
   const int ld = P.npad;
@@ -2200,20 +2202,21 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   float *Jf = reinterpret_cast<float *>(J);    // rows 0..ma-1 in FP32
   if (tid < kTilePts && p0 + tid < P.n_points) {
     const int p = p0 + tid;
-    double t[9];
-    for (int k = 0; k < 9; k++) t[k] = 0;
+    double t[8];
+    for (int k = 0; k < 8; k++) t[k] = 0;
     for (int q = 0; q < kWaves; q++)
-      for (int k = 0; k < 9; k++) t[k] += (double)red[((size_t)q * kTilePts + tid) * 9 + k];
+      for (int k = 0; k < 8; k++) t[k] += (double)red[((size_t)q * kTilePts + tid) * 8 + k];
+    const double br = cld * t[6] + x[P.ma - 1] * t[7];  // sum of Area mu mu0 (cl + cls/(mu+mu0))
     const PointGeom G = gst[tid];
     const double sc = G.scale;
-    const double V[3] = {t[1], t[2], t[3]}, U[3] = {t[4], t[5], t[6]};
+    const double V[3] = {t[0], t[1], t[2]}, U[3] = {t[3], t[4], t[5]};
     double r[3];
     rotation_derivs(P, p, blm, G, V, U, r);
     for (int k = 0; k < 3; k++) Jf[(size_t)(P.ncoef + k) * ld + p] = (float)(sc * r[k]);
-    for (int i = 0; i < 3; i++) Jf[(size_t)(P.ncoef + 3 + i) * ld + p] = (float)(t[0] * G.dphp[i]);
-    Jf[(size_t)(P.ncoef + 6) * ld + p] = (float)(sc * t[7] * cld);
-    Jf[(size_t)(P.ncoef + 7) * ld + p] = (float)(sc * t[8]);
-    J[(size_t)P.ma * ld + p] = t[0] * sc;
+    for (int i = 0; i < 3; i++) Jf[(size_t)(P.ncoef + 3 + i) * ld + p] = (float)(br * G.dphp[i]);
+    Jf[(size_t)(P.ncoef + 6) * ld + p] = (float)(sc * t[6] * cld);
+    Jf[(size_t)(P.ncoef + 7) * ld + p] = (float)(sc * t[7]);
+    J[(size_t)P.ma * ld + p] = br * sc;
   }
 #pragma unroll
   for (int cb = 0; cb < NB; cb++) {
