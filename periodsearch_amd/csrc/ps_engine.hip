@@ -136,7 +136,7 @@ enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_DRAINED = 3 };
 
 // Offsets (in doubles) inside one slot's memory block.
 struct SlotMem {
-  long long J, alpha0, alpha1, beta0, beta1, da, a, atry, alast, area, g, total;
+  long long J, alpha0, alpha1, beta0, beta1, da, a, atry, alast, area, g, perm, pys, psg, psw, total;
 };
 
 __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
@@ -153,6 +153,12 @@ __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
   m.a = o; o += kMaxMa;
   m.atry = o; o += kMaxMa;
   m.alast = o; o += kMaxMa;
+  // mixed forms with ordered points: sorted position -> point (int), and the
+  // points' brightness, sigma and weight/sigma^2 in sorted order
+  m.perm = o; o += (npad + 1) / 2;
+  m.pys = o; o += npad;
+  m.psg = o; o += npad;
+  m.psw = o; o += npad;
   o = (o + 31) / 32 * 32;  // 256-byte aligned blocks
   m.total = o;
   return m;
@@ -160,7 +166,8 @@ __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
 
 // One batch of work: fits (mode 0) or single model evaluations (mode 1).
 struct DevWork {
-  int n_items, mode, nslots, pad;  // mode 0: fits from the reference starts, 1: evaluations, 2: fits from cg_in
+  int n_items, mode, nslots;  // mode 0: fits from the reference starts, 1: evaluations, 2: fits from cg_in
+  int order;  // 1: each fit's points in its own order (point_order; mixed forms with the K-split normal equations)
   double us_per_tick;       // period of the device wall clock [us]
   const double *freq;       // [n_items] (mode 0)
   const int *pole;          // [n_items] (mode 0)
@@ -788,9 +795,10 @@ __device__ __forceinline__ float row_bcast(float v, int i) {
 }
 
 template <int NB>
-__device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const double *J, double conw_r,
-                                                    double *ks, double *dave, double *drat, const int *fidx,
-                                                    double *beta, double *chi_out, double *alpha) {
+__device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const double *J, const double *__restrict__ yod,
+                                                    const double *__restrict__ sgd, const double *__restrict__ swd,
+                                                    double conw_r, double *ks, double *dave, double *drat,
+                                                    const int *fidx, double *beta, double *chi_out, double *alpha) {
 #pragma clang fp contract(off)
   constexpr int NT = NB * (NB + 1) / 2, NC = 16 * NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -834,9 +842,9 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
     const int q = 16 * g + 4 * kq + i4, qc = min(q, npts - 1);
     G.ymq = ym[q];
     if (pts) {
-      G.sgq = P.sig[qc];
-      G.swq = P.swt[qc];
-      G.yoq = P.y[qc];
+      G.sgq = sgd[qc];
+      G.swq = swd[qc];
+      G.yoq = yod[qc];
     }
   };
   for (int lc = 0; lc <= P.n_lc; lc++) {
@@ -1530,11 +1538,68 @@ __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, 
 // registers of four waves (gauss_solve_block); otherwise 256 threads and the
 // LDS solver.
 constexpr int kWaveSolveMax = 63;
-template <bool MULTI, int RW, int BR = 0>  // RW: register rows of the one-wave solver (>= mfit); BR: rows per
-                                           // wave of the four-wave solver (>= mfit / 4); both 0: LDS solver
-__global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) step_kernel(DevProblem P0,
-                                                                        const DevProblem *__restrict__ probs,
-                                                                        DevWork W) {
+// Point order of a fit (mixed forms): within each light curve, the points by
+// the body-frame longitude of the Earth-Sun bisector at the fit's starting
+// pole and period, in kOrderBins bins, ties in point order (a stable
+// counting sort, so the order is deterministic).  Points that see nearly
+// the same facets then share a model tile, and the model kernel's wave-wide
+// skip of facets that no point of the wave sees takes effect more often.
+// Only the order in which the normal equations sum the points changes.
+// Also writes the points' brightness, sigma and weight/sigma^2 in that
+// order for the normal equations.  lds: 16 doubles + kOrderBins + 1 ints.
+constexpr int kOrderBins = 64;
+__device__ void point_order(const DevProblem &P, const SlotMem &M, double *mem, int tid, int nthreads, double *lds) {
+  const double *a = mem + M.a;
+  int *perm = reinterpret_cast<int *>(mem + M.perm);
+  int *bin = reinterpret_cast<int *>(mem + M.psw);  // bins first; the weights overwrite them at the end
+  double *blm = lds;
+  int *cnt = reinterpret_cast<int *>(lds + 16);
+  const int nc = P.ncoef, n = P.n_points;
+  if (tid == 0) eval_blmatrix(a[nc], a[nc + 1], blm);
+  __syncthreads();
+  const double om = a[nc + 2];
+  for (int p = tid; p < n; p += nthreads) {
+    const double bx = P.ex[p] + P.sx[p], by = P.ey[p] + P.sy[p], bz = P.ez[p] + P.sz[p];
+    const double ux = blm[0] * bx + blm[1] * by + blm[2] * bz, uy = blm[3] * bx + blm[4] * by + blm[5] * bz;
+    double lon = atan2(uy, ux) - fmod(om * P.tim[p] + P.phi0, 2 * kPi);  // body-frame longitude
+    lon -= 2 * kPi * floor(lon / (2 * kPi));
+    bin[p] = min(max((int)(lon * (kOrderBins / (2 * kPi))), 0), kOrderBins - 1);
+  }
+  __syncthreads();
+  for (int lc = 0; lc < P.n_lc; lc++) {
+    const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1];
+    if (tid < kOrderBins) {
+      int c = 0;
+      for (int q = q0; q < q1; q++) c += bin[q] == tid ? 1 : 0;
+      cnt[tid + 1] = c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      cnt[0] = 0;
+      for (int b = 0; b < kOrderBins; b++) cnt[b + 1] += cnt[b];
+    }
+    __syncthreads();
+    if (tid < kOrderBins) {
+      int k = q0 + cnt[tid];
+      for (int q = q0; q < q1; q++)
+        if (bin[q] == tid) perm[k++] = q;
+    }
+    __syncthreads();
+  }
+  double *pys = mem + M.pys, *psg = mem + M.psg, *psw = mem + M.psw;
+  for (int q = tid; q < n; q += nthreads) {
+    const int p = perm[q];
+    pys[q] = P.y[p];
+    psg[q] = P.sig[p];
+  }
+  __syncthreads();
+  for (int q = tid; q < n; q += nthreads) psw[q] = P.swt[perm[q]];
+  __syncthreads();
+}
+
+template <bool MULTI, int RW, int BR>  // RW: register rows of the one-wave solver (>= mfit); BR: rows per
+                                       // wave of the four-wave solver (>= mfit / 4); both 0: LDS solver
+__device__ __forceinline__ void step_body(const DevProblem &P0, const DevProblem *__restrict__ probs, const DevWork &W) {
   constexpr bool WAVE = RW > 0;
   constexpr bool BLOCK = BR > 0;
   constexpr int kStepThreads = WAVE ? 64 : kThreads;
@@ -1633,6 +1698,7 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) ste
       S.err = 0;
     }
     __syncthreads();
+    if (W.order) point_order(P, M, mem, tid, kStepThreads, sm);
     slot_curv(P, M, mem, tid, kStepThreads);
     return;
   }
@@ -1726,6 +1792,13 @@ __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) ste
     S.eval = err == 0 ? 1 : 0;
     S.err = err;
   }
+}
+
+template <bool MULTI, int RW, int BR = 0>
+__global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) step_kernel(DevProblem P0,
+                                                                        const DevProblem *__restrict__ probs,
+                                                                        DevWork W) {
+  step_body<MULTI, RW, BR>(P0, probs, W);
 }
 
 // model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
@@ -2026,9 +2099,13 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   for (int f = tid; f < P.fpad; f += kThreads) areaf[f] = f < F ? (float)mem[M.area + f] : 0.0f;
   if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
   __syncthreads();
+  // tile positions p0..p0+63: points in the fit's order (point_order) or in
+  // input order; the Jacobian's columns follow the positions
   const int p0 = tile * kTilePts;
+  const int *__restrict__ perm = reinterpret_cast<const int *>(mem + M.perm);
   if (tid < kTilePts) {
-    const int p = min(p0 + tid, P.n_points - 1);
+    const int q = min(p0 + tid, P.n_points - 1);
+    const int p = W.order ? perm[q] : q;
     PointGeom G;
     point_geometry(P, p, x, blm, G);
     gst[tid] = G;
@@ -2201,7 +2278,7 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   double *J = mem + M.J;                       // row ma (ymod) in FP64
   float *Jf = reinterpret_cast<float *>(J);    // rows 0..ma-1 in FP32
   if (tid < kTilePts && p0 + tid < P.n_points) {
-    const int p = p0 + tid;
+    const int p = p0 + tid, pt = W.order ? perm[p] : p;  // position, point
     double t[8];
     for (int k = 0; k < 8; k++) t[k] = 0;
     for (int q = 0; q < kWaves; q++)
@@ -2211,7 +2288,7 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
     const double sc = G.scale;
     const double V[3] = {t[0], t[1], t[2]}, U[3] = {t[3], t[4], t[5]};
     double r[3];
-    rotation_derivs(P, p, blm, G, V, U, r);
+    rotation_derivs(P, pt, blm, G, V, U, r);
     for (int k = 0; k < 3; k++) Jf[(size_t)(P.ncoef + k) * ld + p] = (float)(sc * r[k]);
     for (int i = 0; i < 3; i++) Jf[(size_t)(P.ncoef + 3 + i) * ld + p] = (float)(br * G.dphp[i]);
     Jf[(size_t)(P.ncoef + 6) * ld + p] = (float)(sc * t[6] * cld);
@@ -2237,8 +2314,8 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
 #ifndef PS_NEQ_KS_WAVES
 #define PS_NEQ_KS_WAVES 2
 #endif
-template <int NT, bool MIX, bool MULTI, int KS = 0>
-__global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+template <int NT, bool MIX, bool MULTI, int KS>
+__device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem *__restrict__ probs, const DevWork &W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
   SlotState &S = W.st[slot];
@@ -2263,8 +2340,10 @@ __global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_ke
     __threadfence_block();
     __syncthreads();
     if constexpr (KS > 0)
-      normal_equations_ks<KS>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx, btr,
-                              scal, atr);
+      // the points' brightness, sigma and weight/sigma^2 in the Jacobian's column order
+      normal_equations_ks<KS>(P, mem + M.J, W.order ? mem + M.pys : P.y, W.order ? mem + M.psg : P.sig,
+                              W.order ? mem + M.psw : P.swt, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave,
+                              drat, fidx, btr, scal, atr);
     else
       normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx,
                                      btr, scal, atr);
@@ -2354,6 +2433,26 @@ __global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_ke
     S.eval = 0;
   }
   (void)atry;
+}
+
+template <int NT, bool MIX, bool MULTI, int KS = 0>
+__global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+  neq_body<NT, MIX, MULTI, KS>(P0, probs, W);
+}
+
+// iter_kernel: one workgroup per slot, the end of one LM iteration and the
+// start of the next in one launch: the normal equations at the trial
+// parameters with accept/reject and the driver bookkeeping (neq_body), then
+// the loop predicate, the fit summary or refill, and the damped solve with
+// the trial parameters and their shape (step_body).  For more than 63 free
+// parameters, where both parts run on four waves (the four-wave register
+// solver, BR rows per wave, or the LDS solver, BR = 0).  An LM iteration is
+// then two launches: this kernel and the model kernel.
+template <int NT, bool MIX, bool MULTI, int KS, int BR>
+__global__ void __launch_bounds__(kThreads, 2) iter_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+  neq_body<NT, MIX, MULTI, KS>(P0, probs, W);
+  __syncthreads();
+  step_body<MULTI, 0, BR>(P0, probs, W);
 }
 
 // Device memory of ps_fit_batch_noalloc: counters, slot states and slot
@@ -2586,6 +2685,24 @@ NeqFn pick_neq(int mfit, int ks) {
   return nullptr;
 }
 
+// The fused iteration kernel for mfit > 63 free parameters (the solver's
+// rows per wave from mfit, the normal equations as pick_neq chooses them).
+template <bool MIX, bool MULTI>
+NeqFn pick_iter(int mfit, int ks) {
+  if (mfit <= kWaveSolveMax || mfit > kMaxMfit) return nullptr;
+  if (MIX && ks == 4 && mfit <= 64) return iter_kernel<1, MIX, MULTI, 4, 16>;
+  if (MIX && ks == 5) return iter_kernel<1, MIX, MULTI, 5, 24>;
+  if (MIX && ks == 6) return iter_kernel<1, MIX, MULTI, 6, 24>;
+  if (ks == 0) {
+    const int nb = (mfit + 15) / 16;
+    if (nb == 4) return iter_kernel<3, MIX, MULTI, 0, 16>;
+    if (nb <= 6) return iter_kernel<6, MIX, MULTI, 0, 24>;
+    if (mfit <= kBlockSolveMax) return iter_kernel<9, MIX, MULTI, 0, 32>;
+    return iter_kernel<9, MIX, MULTI, 0, 0>;
+  }
+  return nullptr;
+}
+
 
 // The kernels of one LM iteration, chosen for a set of problems: their
 // largest coefficient count and free-parameter count pick the instances,
@@ -2594,7 +2711,9 @@ struct KernelSet {
   ModelFn model = nullptr;
   NeqFn neq = nullptr;
   StepFn step = nullptr;
-  size_t lds_step = 0, lds_model = 0, lds_neq = 0;
+  NeqFn iter = nullptr;  // neq + step in one launch (more than 63 free parameters), or null
+  size_t lds_step = 0, lds_model = 0, lds_neq = 0, lds_iter = 0;
+  int order = 0;         // DevWork.order of the set's batches
   int step_threads = kThreads;
   bool use_mfma = true;
   DevProblem G{};  // maxima over the set's problems
@@ -2650,6 +2769,12 @@ struct ps_engine {
   void (*step)(DevProblem, const DevProblem *, DevWork) = nullptr;
   bool use_mfma = true;   // FP64: matrix-core contraction (default) or the VALU kernel
   bool neq_staged = false;  // mixed forms: the staged normal-equation kernel instead of the K-split one (PSAMD_NEQ=staged)
+  bool fuse = false;        // neq + step in one launch where both run on four waves (PSAMD_FUSE=1); measured
+                            // slower than the two launches (register pressure), so off by default
+  NeqFn iter = nullptr;
+  size_t lds_iter = 0;
+  bool order_points = true;  // mixed forms: each fit's points ordered for the model kernel (PSAMD_ORDER=0: input order)
+  int order = 0;
   // slot memory
   SlotState *st = nullptr;
   double *mem = nullptr;
@@ -2734,6 +2859,7 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   ks.lds_step = step_lds(reg_solve ? 0 : mfit, F);
   ks.lds_model = mixed ? mixed_layout(F, nb, mode).total : (ks.use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
   const int neq_blocks = neq_ks_blocks(mfit, mixed, e->neq_staged);
+  ks.order = neq_blocks > 0 && e->order_points ? 1 : 0;
   ks.lds_neq = neq_lds(mfit, mixed, neq_blocks);
   const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
   if (ks.lds_step > lds_max || ks.lds_neq > lds_max || ks.lds_model > lds_max) {
@@ -2742,6 +2868,11 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   }
   ks.step = multi ? pick_step<true>(mfit) : pick_step<false>(mfit);
   ks.step_threads = wave_solve ? 64 : kThreads;
+  if (e->fuse) {
+    ks.iter = mixed ? (multi ? pick_iter<true, true>(mfit, neq_blocks) : pick_iter<true, false>(mfit, neq_blocks))
+                    : (multi ? pick_iter<false, true>(mfit, 0) : pick_iter<false, false>(mfit, 0));
+    ks.lds_iter = std::max(ks.lds_step, ks.lds_neq);
+  }
   ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit, neq_blocks) : pick_neq<true, false>(mfit, neq_blocks))
                  : (multi ? pick_neq<false, true>(mfit, 0) : pick_neq<false, false>(mfit, 0));
   if (!ks.neq) {
@@ -2754,6 +2885,9 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
                               (int)ks.lds_neq));
   HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(ks.model), hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ks.lds_model));
+  if (ks.iter)
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(ks.iter), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ks.lds_iter));
   return PS_OK;
 }
 
@@ -2860,6 +2994,9 @@ int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
   eks.model = e->model;
   eks.neq = e->neq;
   eks.step = e->step;
+  eks.iter = e->iter;
+  eks.lds_iter = e->lds_iter;
+  eks.order = e->order;
   eks.lds_step = e->lds_step;
   eks.lds_model = e->lds_model;
   eks.lds_neq = e->lds_neq;
@@ -2876,6 +3013,7 @@ int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
       Lane &L = lane[nl];
       L.ks = &e->cls[c];
       L.W = W;
+      L.W.order = e->cls[c].order;
       L.W.n_items = n_c;
       const int start = e->cls_start[c];
       L.W.freq = W.freq + start;
@@ -2895,6 +3033,7 @@ int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
     for (int g = 0; g < ng; g++) {
       lane[g].ks = &eks;
       lane[g].W = W;
+      lane[g].W.order = eks.order;
       lane[g].W.nslots = nslots / ng + (g < nslots % ng ? 1 : 0);
       lane[g].W.queue = counters;
       lane[g].W.done = counters + 1;
@@ -2942,12 +3081,16 @@ int run_batch(ps_engine *e, DevWork W, const FixedSlots *fixed = nullptr) {
         const DevWork &Wk = lane[l].W;
         const KernelSet &K = *lane[l].ks;
         const int ns = Wk.nslots;
-        hipLaunchKernelGGL(K.step, dim3(ns), dim3(K.step_threads), K.lds_step, sg[l], P0, probs, Wk);
+        // step, model, neq; or (fused) neq + step, then model
+        if (K.iter)
+          hipLaunchKernelGGL(K.iter, dim3(ns), dim3(kThreads), K.lds_iter, sg[l], P0, probs, Wk);
+        else
+          hipLaunchKernelGGL(K.step, dim3(ns), dim3(K.step_threads), K.lds_step, sg[l], P0, probs, Wk);
         if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + l)], sg[l]));
         hipLaunchKernelGGL(K.model, dim3(ns * (K.use_mfma ? K.G.ntiles64 : K.G.ntiles)), dim3(kThreads),
                            K.lds_model, sg[l], P0, probs, Wk);
         if (prof) HIP_TRY(hipEventRecord(e->pev[2 * (kMaxGroups * k + l) + 1], sg[l]));
-        hipLaunchKernelGGL(K.neq, dim3(ns), dim3(kThreads), K.lds_neq, sg[l], P0, probs, Wk);
+        if (!K.iter) hipLaunchKernelGGL(K.neq, dim3(ns), dim3(kThreads), K.lds_neq, sg[l], P0, probs, Wk);
       }
     }
     HIP_TRY(hipGetLastError());
@@ -3336,6 +3479,8 @@ int ps_create(int device, ps_engine **out) {
   }
   if (const char *gr = getenv("PSAMD_GRAPH")) e->use_graph = atoi(gr) != 0;
   if (const char *nq = getenv("PSAMD_NEQ")) e->neq_staged = strcmp(nq, "staged") == 0;
+  if (const char *fu = getenv("PSAMD_FUSE")) e->fuse = atoi(fu) != 0;
+  if (const char *po = getenv("PSAMD_ORDER")) e->order_points = atoi(po) != 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
     e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);
@@ -3475,6 +3620,9 @@ int ps_set_problems(ps_engine *e, const ps_problem *p, int n) {
   e->model = ks.model;
   e->neq = ks.neq;
   e->step = ks.step;
+  e->iter = ks.iter;
+  e->lds_iter = ks.lds_iter;
+  e->order = ks.order;
   e->lds_step = ks.lds_step;
   e->lds_model = ks.lds_model;
   e->lds_neq = ks.lds_neq;

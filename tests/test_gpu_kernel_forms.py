@@ -1,0 +1,114 @@
+"""Alternative kernel forms of the mixed-precision LM iteration against each
+other, at the 5,000-point shape (lmax 8, 800 facets, 87 free parameters):
+
+* the normal equations split over the waves by points (default for the
+  mixed forms) against the staged kernel (PSAMD_NEQ=staged): one evaluation
+  agrees to the FP32 summation-order level, and both agree with the oracle;
+* each fit's points in their own visibility order (default for the mixed
+  forms) against input order (PSAMD_ORDER=0): the same model values and
+  derivatives, summed in another order;
+* the normal equations and the solve in one launch (PSAMD_FUSE=1) against
+  two launches: bit for bit the same fits, in FP64 and in a mixed form.
+
+The switches are read when an engine is created, so each engine below is
+made with the environment it needs."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def big(tmp_path_factory):
+    import oracle
+    from periodsearch_amd import Input, synth
+
+    path = str(tmp_path_factory.mktemp("forms") / "period_search_in")
+    ast = synth.write_workload("big", path, incl_deg=10.0)
+    inp = Input(path)
+    kt = round((inp.freq_start - 1 / ast["period_h"]) / inp.freq_step) + 1
+    orc = oracle.Oracle(path)
+    orc.prepare(inp.conw_r0)
+    return inp, orc, kt
+
+
+def _engine(inp, prec, **env):
+    from periodsearch_amd import Engine
+
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        eng = Engine(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    eng.set_precision(prec)
+    eng.set_problem(inp.problem, keep=inp)
+    return eng
+
+
+def _starts(inp, orc, kt):
+    nc = orc.iget("Ncoef")
+    rng = np.random.default_rng(3)
+    cgs = []
+    for pole in (1, 4, 7, 10):
+        cg1 = orc.initial_cg(inp.freq_start - (kt - 1) * inp.freq_step, pole)
+        cg1[nc + 1:nc + 7] += rng.normal(0, 1e-3, size=6) * np.maximum(np.abs(cg1[nc + 1:nc + 7]), 0.05)
+        cgs.append(cg1)
+    return cgs
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x6"])
+def test_split_and_staged_normal_equations_agree(big, form):
+    from periodsearch_amd import PS_PREC_BF16X6, PS_PREC_FP32
+
+    inp, orc, kt = big
+    prec = {"fp32": PS_PREC_FP32, "bf16x6": PS_PREC_BF16X6}[form]
+    cgs = _starts(inp, orc, kt)
+    cg = np.array([c[1:] for c in cgs])
+    outs = {}
+    for name, env in (("split", {}), ("staged", {"PSAMD_NEQ": "staged"}), ("input order", {"PSAMD_ORDER": "0"})):
+        eng = _engine(inp, prec, **env)
+        outs[name] = eng.eval_mrqcof(cg, inp.conw_r0)[:3]
+        eng.close()
+    for i, cg1 in enumerate(cgs):
+        A, b, c = orc.mrqcof(cg1)
+        for name, (alpha, beta, chi) in outs.items():
+            ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
+            eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+            ec = abs(chi[i] - c) / c
+            print(f"{form} {name} start {i}: alpha {ea:.2e} beta {eb:.2e} chi2 {ec:.2e} against the oracle")
+            # the FP32 facet stage bounds every form (about 3e-7 alpha, 2e-6 beta)
+            assert ea < 1e-5 and eb < 1e-4 and ec < 1e-6, (name, i, ea, eb, ec)
+        a0, b0, c0 = (outs["split"][k][i] for k in range(3))
+        for name in ("staged", "input order"):
+            a1, b1, c1 = (outs[name][k][i] for k in range(3))
+            # the same FP32 Jacobian summed in another order
+            assert np.abs(a1 - a0).max() / np.abs(a0).max() < 2e-6, name
+            assert np.abs(b1 - b0).max() / np.abs(b0).max() < 2e-5, name
+            assert abs(c1 - c0) / c0 < 1e-7, name
+
+
+@pytest.mark.parametrize("form", ["fp64", "bf16"])
+def test_fused_iteration_kernel_equals_two_launches(big, form):
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_FP64
+
+    inp, orc, kt = big
+    prec = {"fp64": PS_PREC_FP64, "bf16": PS_PREC_BF16}[form]
+    freq = [inp.freq_start - (kt - 1 + d) * inp.freq_step for d in (0, 1)]
+    freqs = [f for f in freq for _ in range(10)]
+    poles = list(range(10)) * 2
+    res = {}
+    for fuse in (0, 1):
+        eng = _engine(inp, prec, PSAMD_FUSE=fuse)
+        res[fuse] = eng.fit_batch(freqs, poles, inp.conw_r0)
+        eng.close()
+    from periodsearch_amd import FIT_RESULT_FIELDS
+
+    for k in FIT_RESULT_FIELDS:
+        assert np.array_equal(res[0][k], res[1][k]), k
