@@ -1500,6 +1500,104 @@ __device__ __forceinline__ int gauss_solve_block(double (&a0)[RB], double (&a1)[
   return 0;
 }
 
+// Gauss-Jordan with the pivots on the diagonal (the mixed forms' solve, in
+// FP64): each pivot step does gauss_solve_block's arithmetic, with the pivot
+// the not-yet-pivoted row j of largest |a_jj|, ties to the larger j (the
+// reference's ">=" row-major scan restricted to the diagonal).  The damped
+// normal matrices of a fit are symmetric and positive definite in exact
+// arithmetic, so |a_jk| <= sqrt(a_jj a_kk): complete pivoting
+// (gauss_errc.c:21-36) finds its maximum on the diagonal as well, up to
+// rounding-level ties.  Diagonal pivots need no row swaps, and each step
+// reduces n diagonal entries instead of the whole remaining matrix.
+// Layout as gauss_solve_block: wave w holds physical = logical rows
+// w + 4r, lane l columns l and l + 64.  LDS: prow [kMaxMfit + 1], dg
+// [kMaxMfit + 1] (the current diagonal).  Returns 2 on a zero pivot, 1 when
+// no candidate is left (NaN diagonal), as gauss_errc reports singular
+// matrices.
+template <int RB>
+__device__ __forceinline__ int gauss_solve_block_diag(double (&a0)[RB], double (&a1)[RB], int n, double *x,
+                                                      double *prow, double *dg) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = lane, c1 = lane + 64;
+  unsigned long long done0 = 0ull, done1 = 0ull;  // pivoted rows (= columns)
+  // the diagonal entries of this wave's rows, in the lanes of their columns
+  auto put_diag = [&]() __attribute__((always_inline)) {
+    double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int p = wave + 4 * r;
+      if (r < 16)
+        d0 = (lane == p) ? a0[r] : d0;
+      else
+        d1 = (lane == p - 64) ? a1[r] : d1;
+    }
+    if ((lane & 3) == wave) {
+      if (c0 < n) dg[c0] = d0;
+      if (c1 < n) dg[c1] = d1;
+    }
+  };
+  // the next pivot row: argmax of |a_jj| over the rows not pivoted yet
+  auto pick = [&]() __attribute__((always_inline)) {
+    double bv = -1.0;
+    int be = -1;
+    if (c0 < n && !((done0 >> c0) & 1ull)) {
+      bv = fabs(dg[c0]);
+      be = c0;
+    }
+    if (c1 < n && !((done1 >> lane) & 1ull)) pivot_max(bv, be, fabs(dg[c1]), c1);
+    wave_pivot_max(bv, be);
+    return be;
+  };
+  put_diag();
+  __syncthreads();
+  int icol = pick();
+  for (int i = 0; i < n; i++) {
+    if (icol < 0) return 1;
+    if (wave == (icol & 3)) {
+      const int rp = icol >> 2;
+      double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+      for (int r = 0; r < RB; r++) {
+        v0 = (r == rp) ? a0[r] : v0;
+        v1 = (r == rp) ? a1[r] : v1;
+      }
+      if (c0 <= n) prow[c0] = v0;
+      if (c1 <= n) prow[c1] = v1;
+    }
+    __syncthreads();
+    const double piv = prow[icol];
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    if (icol < 64)
+      done0 |= 1ull << icol;
+    else
+      done1 |= 1ull << (icol - 64);
+    const double pr0 = prow[min(c0, n)] * pivinv, pr1 = prow[min(c1, n)] * pivinv;  // new a[icol][c]
+    const int hi = icol >= 64, il = icol & 63;
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int p = wave + 4 * r;
+      const double dum = lane_value(hi ? a1[r] : a0[r], il);
+      a0[r] = (p == icol) ? pr0 : a0[r] - pr0 * dum;
+      a1[r] = (p == icol) ? pr1 : a1[r] - pr1 * dum;
+    }
+    put_diag();
+    __syncthreads();
+    icol = pick();
+  }
+  const int bh = n >= 64, bl = n & 63;  // column n (b)
+#pragma unroll
+  for (int r = 0; r < RB; r++) {
+    const int p = wave + 4 * r;
+    if (p < n) {
+      const double bp = lane_value(bh ? a1[r] : a0[r], bl);
+      if (lane == 0) x[p] = bp;
+    }
+  }
+  return 0;
+}
+
 __device__ void slot_curv(const DevProblem &P, const SlotMem &M, double *mem, int tid, int nthreads) {
 #pragma clang fp contract(off)
   const double *atry = mem + M.atry;
@@ -1597,8 +1695,10 @@ __device__ void point_order(const DevProblem &P, const SlotMem &M, double *mem, 
   __syncthreads();
 }
 
-template <bool MULTI, int RW, int BR>  // RW: register rows of the one-wave solver (>= mfit); BR: rows per
-                                       // wave of the four-wave solver (>= mfit / 4); both 0: LDS solver
+template <bool MULTI, int RW, int BR, bool DIAG = false>  // RW: register rows of the one-wave solver (>= mfit); BR:
+                                                         // rows per wave of the four-wave solver (>= mfit / 4); both
+                                                         // 0: LDS solver; DIAG: the four-wave solver with diagonal
+                                                         // pivots (mixed forms)
 __device__ __forceinline__ void step_body(const DevProblem &P0, const DevProblem *__restrict__ probs, const DevWork &W) {
   constexpr bool WAVE = RW > 0;
   constexpr bool BLOCK = BR > 0;
@@ -1744,7 +1844,10 @@ __device__ __forceinline__ void step_body(const DevProblem &P0, const DevProblem
       a0[r] = va * ((p == ca) ? (1 + lam) : 1.0);  // times 1.0 is exact; beta's column never matches
       a1[r] = vb * ((p == cb) ? (1 + lam) : 1.0);
     }
-    err = gauss_solve_block<BLOCK ? BR : 1>(a0, a1, mfit, da, sm, sm + 128);
+    if constexpr (DIAG)
+      err = gauss_solve_block_diag<BLOCK ? BR : 1>(a0, a1, mfit, da, sm, sm + 128);
+    else
+      err = gauss_solve_block<BLOCK ? BR : 1>(a0, a1, mfit, da, sm, sm + 128);
     __syncthreads();
   } else {
   {
@@ -1794,11 +1897,11 @@ __device__ __forceinline__ void step_body(const DevProblem &P0, const DevProblem
   }
 }
 
-template <bool MULTI, int RW, int BR = 0>
+template <bool MULTI, int RW, int BR = 0, bool DIAG = false>
 __global__ void __launch_bounds__(RW ? 64 : kThreads, RW ? 1 : (BR ? 2 : 4)) step_kernel(DevProblem P0,
                                                                         const DevProblem *__restrict__ probs,
                                                                         DevWork W) {
-  step_body<MULTI, RW, BR>(P0, probs, W);
+  step_body<MULTI, RW, BR, DIAG>(P0, probs, W);
 }
 
 // model_kernel: one workgroup per (slot, tile); blockIdx.x = tile * nslots + slot.
@@ -2621,7 +2724,7 @@ size_t mfma_lds(int nfac, int ncoef) {
 
 size_t step_lds(int mfit, int nfac) {  // mfit = 0: a register solver (at most 136 doubles of LDS)
   if (mfit == 0) {
-    const size_t fin = 2 * (size_t)nfac, reg = 128 + 2 * kWaves;
+    const size_t fin = 2 * (size_t)nfac, reg = 2 * (kMaxMfit + 1) + 2 * kWaves;
     return (fin > reg ? fin : reg) * sizeof(double);
   }
   const size_t copies = solve_in_place(mfit) ? 1 : 2;
@@ -2640,12 +2743,12 @@ using StepFn = void (*)(DevProblem, const DevProblem *, DevWork);
 // Solver kernel for mfit free parameters: the one-wave register solver with
 // the smallest row count that holds them, or (mfit = 0 here) the LDS solver.
 template <bool MULTI>
-StepFn pick_step(int mfit) {
+StepFn pick_step(int mfit, bool diag) {
   if (mfit <= 0 || mfit > kBlockSolveMax) return step_kernel<MULTI, 0>;
   if (mfit > kWaveSolveMax) {
-    if (mfit <= 64) return step_kernel<MULTI, 0, 16>;
-    if (mfit <= 96) return step_kernel<MULTI, 0, 24>;
-    return step_kernel<MULTI, 0, 32>;
+    if (mfit <= 64) return diag ? step_kernel<MULTI, 0, 16, true> : step_kernel<MULTI, 0, 16>;
+    if (mfit <= 96) return diag ? step_kernel<MULTI, 0, 24, true> : step_kernel<MULTI, 0, 24>;
+    return diag ? step_kernel<MULTI, 0, 32, true> : step_kernel<MULTI, 0, 32>;
   }
   if (mfit <= 16) return step_kernel<MULTI, 16>;
   if (mfit <= 24) return step_kernel<MULTI, 24>;
@@ -2774,6 +2877,7 @@ struct ps_engine {
   NeqFn iter = nullptr;
   size_t lds_iter = 0;
   bool order_points = true;  // mixed forms: each fit's points ordered for the model kernel (PSAMD_ORDER=0: input order)
+  bool diag_pivots = true;   // mixed forms: diagonal pivots in the four-wave solve (PSAMD_PIVOT=full: complete pivoting)
   int order = 0;
   // slot memory
   SlotState *st = nullptr;
@@ -2866,7 +2970,8 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;
   }
-  ks.step = multi ? pick_step<true>(mfit) : pick_step<false>(mfit);
+  const bool diag = mixed && e->diag_pivots;
+  ks.step = multi ? pick_step<true>(mfit, diag) : pick_step<false>(mfit, diag);
   ks.step_threads = wave_solve ? 64 : kThreads;
   if (e->fuse) {
     ks.iter = mixed ? (multi ? pick_iter<true, true>(mfit, neq_blocks) : pick_iter<true, false>(mfit, neq_blocks))
@@ -3481,6 +3586,7 @@ int ps_create(int device, ps_engine **out) {
   if (const char *nq = getenv("PSAMD_NEQ")) e->neq_staged = strcmp(nq, "staged") == 0;
   if (const char *fu = getenv("PSAMD_FUSE")) e->fuse = atoi(fu) != 0;
   if (const char *po = getenv("PSAMD_ORDER")) e->order_points = atoi(po) != 0;
+  if (const char *pv = getenv("PSAMD_PIVOT")) e->diag_pivots = strcmp(pv, "full") != 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
     e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);

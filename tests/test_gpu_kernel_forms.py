@@ -8,7 +8,10 @@ other, at the 5,000-point shape (lmax 8, 800 facets, 87 free parameters):
   forms) against input order (PSAMD_ORDER=0): the same model values and
   derivatives, summed in another order;
 * the normal equations and the solve in one launch (PSAMD_FUSE=1) against
-  two launches: bit for bit the same fits, in FP64 and in a mixed form.
+  two launches: bit for bit the same fits, in FP64 and in a mixed form;
+* the mixed forms' solve with diagonal pivots (default) against complete
+  pivoting (PSAMD_PIVOT=full): on damped normal matrices the two take the
+  same pivots, so the fits are bit for bit the same.
 
 The switches are read when an engine is created, so each engine below is
 made with the environment it needs."""
@@ -112,3 +115,18 @@ def test_fused_iteration_kernel_equals_two_launches(big, form):
 
     for k in FIT_RESULT_FIELDS:
         assert np.array_equal(res[0][k], res[1][k]), k
+
+
+def test_diagonal_and_complete_pivots_give_the_same_fits(big):
+    from periodsearch_amd import FIT_RESULT_FIELDS, PS_PREC_BF16X6
+
+    inp, orc, kt = big
+    freqs = [inp.freq_start - (kt - 1 + d) * inp.freq_step for d in (-2, 0, 3) for _ in range(10)]
+    poles = list(range(10)) * 3
+    res = {}
+    for pivot in ("diag", "full"):
+        eng = _engine(inp, PS_PREC_BF16X6, PSAMD_PIVOT=pivot)
+        res[pivot] = eng.fit_batch(freqs, poles, inp.conw_r0)
+        eng.close()
+    for k in FIT_RESULT_FIELDS:
+        assert np.array_equal(res["diag"][k], res["full"][k]), k

@@ -118,3 +118,56 @@ def test_block_schedule_singular_codes(n):
     assert code_b == code_r and code_r != 0
     Z = np.zeros((n, n))
     assert block_schedule(Z, b)[0] == reference(Z, b)[0]
+
+
+def diag_schedule(A, b):
+    """gauss_solve_block_diag (the mixed forms' solve): the element updates of
+    gauss_errc with the pivot the not-yet-pivoted row of largest |a_jj|, ties
+    to the larger j, and no row swaps.  Returns (code, x)."""
+    n = len(b)
+    M = np.concatenate([A, b[:, None]], axis=1).astype(np.float64)
+    done = np.zeros(n, bool)
+    for _ in range(n):
+        d = np.abs(np.diagonal(M[:, :n]))
+        ok = ~done & ~np.isnan(d)
+        if not ok.any():
+            return 1, None
+        icol = int(np.flatnonzero(ok & (d == d[ok].max())).max())
+        piv = M[icol, icol]
+        if piv == 0.0:
+            return 2, None
+        pivinv = 1.0 / piv
+        done[icol] = True
+        pr = M[icol] * pivinv
+        dum = M[:, icol].copy()
+        M = M - pr[None, :] * dum[:, None]
+        M[icol] = pr
+    return 0, M[:, n].copy()
+
+
+@pytest.mark.parametrize("n", [64, 65, 87, 100, 127])
+def test_diagonal_pivots_are_gauss_errc_on_normal_matrices(n):
+    # damped normal matrices are symmetric positive definite: complete
+    # pivoting finds its maximum on the diagonal, so the pivot order and
+    # every rounding are gauss_errc's
+    rng = np.random.default_rng(1000 + n)
+    for _ in range(4):
+        A, b = _spd(rng, n), rng.normal(size=n)
+        code_r, x_r = reference(A, b)
+        code_d, x_d = diag_schedule(A, b)
+        assert code_d == code_r == 0
+        assert np.array_equal(x_d, x_r), np.abs(x_d - x_r).max()
+
+
+def test_diagonal_pivots_solve_general_systems():
+    # not positive definite: other pivots than gauss_errc's, the same solution
+    rng = np.random.default_rng(7)
+    for n in (64, 90):
+        A = rng.integers(-3, 4, size=(n, n)).astype(float) + 7 * np.eye(n)
+        b = rng.integers(-5, 6, size=n) * 1.0
+        code_r, x_r = reference(A, b)
+        code_d, x_d = diag_schedule(A, b)
+        assert code_d == code_r == 0
+        assert np.allclose(x_d, x_r, rtol=1e-12, atol=1e-12 * np.abs(x_r).max())
+        assert np.allclose(A @ x_d, b, rtol=0, atol=1e-10 * np.abs(b).max())
+    assert diag_schedule(np.zeros((70, 70)), np.ones(70))[0] == 2
