@@ -60,9 +60,7 @@ HOST_CPU_SHARE = 16  # CPU threads the GPU box allots to one GPU
 # HBM bytes per model evaluation of the model kernel (copy-calibrated
 # FETCH_SIZE + WRITE_SIZE counter runs; profiles/), by workload and precision
 MODEL_HBM_BYTES_PER_EVAL = {
-    ("wu", "fp64"): 469.4e3, ("wu", "fp32"): 242.2e3, ("wu", "bf16"): 242.2e3, ("wu", "bf16x3"): 242.2e3,
-    ("wu", "bf16x6"): 242.2e3,
-    ("c2", "fp64"): 4186.9e3, ("c2", "bf16x6"): 2316.6e3, ("c2", "bf16"): 2270.8e3,
+    ("c2", "fp64"): 4186.0e3, ("c2", "bf16"): 2339.5e3,
 }
 PREC_NAMES = ("fp64", "fp32", "bf16", "bf16x3", "bf16x6")
 DTYPE = {"fp64": "f64",

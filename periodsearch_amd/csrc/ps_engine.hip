@@ -27,7 +27,8 @@
 // A slot whose fit ends is refilled from the queue in the next step_kernel,
 // so fits with different iteration counts keep every slot busy.  A last
 // kernel reduces the poles of each trial period to one output row
-// (period_search_BOINC.cpp:913-924).
+// (period_search_BOINC.cpp:913-924).  iter_kernel runs neq_kernel's and
+// step_kernel's work in one launch (an option: measured slower than two).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2970,7 +2971,7 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
     e->err = "problem needs more LDS than a CU has";
     return PS_ERR_LIMIT;
   }
-  const bool diag = mixed && e->diag_pivots;
+  const bool diag = mixed && e->diag_pivots;  // FP64 keeps gauss_errc's complete pivoting
   ks.step = multi ? pick_step<true>(mfit, diag) : pick_step<false>(mfit, diag);
   ks.step_threads = wave_solve ? 64 : kThreads;
   if (e->fuse) {

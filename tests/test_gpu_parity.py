@@ -436,11 +436,14 @@ def test_mixed_forms_at_configuration_2_shape(c2_scan, name, prec, tol_alpha, to
     alpha, beta, chi, _ = eng.eval_mrqcof(np.array([c[1:] for c in cgs]), inp.conw_r0)
     rows = eng.scan(inp.freq_start, inp.freq_step, kt - 20, kt + 20, inp.conw_r0)
     eng.close()
+    worst = [0.0, 0.0]
     for i, cg1 in enumerate(cgs):
         A, b, c = orc.mrqcof(cg1)
         ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
         eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+        worst = [max(worst[0], ea), max(worst[1], eb)]
         assert ea < tol_alpha and eb < tol_beta and _rel(chi[i], c) < 1e-6, (i, ea, eb)
+    print(f"{name}: one evaluation against the oracle, worst of {len(cgs)}: alpha {worst[0]:.2e} beta {worst[1]:.2e}")
     k64, k = int(np.argmin(rows64["chisq"])), int(np.argmin(rows["chisq"]))
     step_h = rows64["period_h"][k64] ** 2 * inp.freq_step
     bar = tol_chi2 if tol_chi2 is not None else min(5e-4, max(1e-4, progress))
