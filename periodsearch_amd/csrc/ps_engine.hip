@@ -1366,6 +1366,57 @@ __device__ __forceinline__ int gauss_solve_wave(double (&a)[R], int n, double *x
   return 0;
 }
 
+// gauss_solve_wave with the pivots on the diagonal (mixed forms; as
+// gauss_solve_block_diag): the pivot is the not-yet-pivoted row j of largest
+// |a_jj|, ties to the larger j, and no row moves.  Lane j holds a_jj in
+// register row j, so after each elimination the candidates are one value
+// per lane and one wave reduction finds the next pivot.
+template <int R>
+__device__ __forceinline__ int gauss_solve_wave_diag(double (&a)[R], int n, double *x) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  unsigned long long done = 0ull;
+  auto pick = [&]() __attribute__((always_inline)) {
+    double d = 0.0;
+#pragma unroll
+    for (int p = 0; p < R; p++) d = (p == lane) ? a[p] : d;
+    double bv = -1.0;
+    int be = -1;
+    if (lane < n && !((done >> lane) & 1ull)) {
+      bv = fabs(d);
+      be = lane;
+    }
+    wave_pivot_max(bv, be);
+    return be;
+  };
+  int icol = pick();
+  for (int i = 0; i < n; i++) {
+    if (icol < 0) return 1;
+    double prow = 0.0;
+#pragma unroll
+    for (int p = 0; p < R; p++) prow = (p == icol) ? a[p] : prow;
+    const double piv = lane_value(prow, icol);
+    if (piv == 0.0) return 2;
+    const double pivinv = 1.0 / piv;
+    done |= 1ull << icol;
+    const double pr = prow * pivinv;  // new a[icol][c] (b for c == n)
+#pragma unroll
+    for (int p = 0; p < R; p++) {
+      const double dum = lane_value(a[p], icol);
+      a[p] = (p == icol) ? pr : a[p] - pr * dum;
+    }
+    icol = pick();
+  }
+#pragma unroll
+  for (int p = 0; p < R; p++) {
+    if (p < n) {
+      const double bp = lane_value(a[p], n);
+      if (lane == 0) x[p] = bp;
+    }
+  }
+  return 0;
+}
+
 // gauss_errc (period_search/gauss_errc.c:9-73) for 64 <= n <= 127 by four
 // waves with the augmented matrix [A|b] in registers: lane l holds columns l
 // (a0) and l + 64 (a1); wave w holds the physical rows p = w + 4 r, r <
@@ -1827,7 +1878,10 @@ __device__ __forceinline__ void step_body(const DevProblem &P0, const DevProblem
       const double av = colp[j * rstride];
       am[p] = av * ((p == lane) ? (1 + lam) : 1.0);  // times 1.0 is exact; beta's lane never matches
     }
-    err = gauss_solve_wave(am, mfit, da);
+    if constexpr (DIAG)
+      err = gauss_solve_wave_diag(am, mfit, da);
+    else
+      err = gauss_solve_wave(am, mfit, da);
     __syncthreads();
   } else if constexpr (BLOCK) {
 #pragma clang fp contract(off)
@@ -2751,13 +2805,13 @@ StepFn pick_step(int mfit, bool diag) {
     if (mfit <= 96) return diag ? step_kernel<MULTI, 0, 24, true> : step_kernel<MULTI, 0, 24>;
     return diag ? step_kernel<MULTI, 0, 32, true> : step_kernel<MULTI, 0, 32>;
   }
-  if (mfit <= 16) return step_kernel<MULTI, 16>;
-  if (mfit <= 24) return step_kernel<MULTI, 24>;
-  if (mfit <= 32) return step_kernel<MULTI, 32>;
-  if (mfit <= 40) return step_kernel<MULTI, 40>;
-  if (mfit <= 48) return step_kernel<MULTI, 48>;
-  if (mfit <= 56) return step_kernel<MULTI, 56>;
-  return step_kernel<MULTI, 64>;
+  if (mfit <= 16) return diag ? step_kernel<MULTI, 16, 0, true> : step_kernel<MULTI, 16>;
+  if (mfit <= 24) return diag ? step_kernel<MULTI, 24, 0, true> : step_kernel<MULTI, 24>;
+  if (mfit <= 32) return diag ? step_kernel<MULTI, 32, 0, true> : step_kernel<MULTI, 32>;
+  if (mfit <= 40) return diag ? step_kernel<MULTI, 40, 0, true> : step_kernel<MULTI, 40>;
+  if (mfit <= 48) return diag ? step_kernel<MULTI, 48, 0, true> : step_kernel<MULTI, 48>;
+  if (mfit <= 56) return diag ? step_kernel<MULTI, 56, 0, true> : step_kernel<MULTI, 56>;
+  return diag ? step_kernel<MULTI, 64, 0, true> : step_kernel<MULTI, 64>;
 }
 
 using NeqFn = void (*)(DevProblem, const DevProblem *, DevWork);

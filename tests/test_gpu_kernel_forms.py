@@ -10,8 +10,9 @@ other, at the 5,000-point shape (lmax 8, 800 facets, 87 free parameters):
 * the normal equations and the solve in one launch (PSAMD_FUSE=1) against
   two launches: bit for bit the same fits, in FP64 and in a mixed form;
 * the mixed forms' solve with diagonal pivots (default) against complete
-  pivoting (PSAMD_PIVOT=full): on damped normal matrices the two take the
-  same pivots, so the fits are bit for bit the same.
+  pivoting (PSAMD_PIVOT=full), in the four-wave and the one-wave solver: on
+  damped normal matrices the two take the same pivots, so the fits are bit
+  for bit the same.
 
 The switches are read when an engine is created, so each engine below is
 made with the environment it needs."""
@@ -117,10 +118,18 @@ def test_fused_iteration_kernel_equals_two_launches(big, form):
         assert np.array_equal(res[0][k], res[1][k]), k
 
 
-def test_diagonal_and_complete_pivots_give_the_same_fits(big):
-    from periodsearch_amd import FIT_RESULT_FIELDS, PS_PREC_BF16X6
+@pytest.mark.parametrize("workload", ["big", "wu"])
+def test_diagonal_and_complete_pivots_give_the_same_fits(big, tmp_path, workload):
+    # 87 free parameters: the four-wave solver; 55 (the work-unit shape): the one-wave solver
+    from periodsearch_amd import FIT_RESULT_FIELDS, PS_PREC_BF16X6, Input, synth
 
-    inp, orc, kt = big
+    if workload == "big":
+        inp, orc, kt = big
+    else:
+        path = str(tmp_path / "period_search_in")
+        ast = synth.write_workload("wu", path)
+        inp = Input(path)
+        kt = round((inp.freq_start - 1 / ast["period_h"]) / inp.freq_step) + 1
     freqs = [inp.freq_start - (kt - 1 + d) * inp.freq_step for d in (-2, 0, 3) for _ in range(10)]
     poles = list(range(10)) * 3
     res = {}

@@ -121,7 +121,8 @@ def test_block_schedule_singular_codes(n):
 
 
 def diag_schedule(A, b):
-    """gauss_solve_block_diag (the mixed forms' solve): the element updates of
+    """gauss_solve_wave_diag and gauss_solve_block_diag (the mixed forms'
+    solves, one wave up to 63 unknowns, four waves from 64): the element updates of
     gauss_errc with the pivot the not-yet-pivoted row of largest |a_jj|, ties
     to the larger j, and no row swaps.  Returns (code, x)."""
     n = len(b)
@@ -145,7 +146,7 @@ def diag_schedule(A, b):
     return 0, M[:, n].copy()
 
 
-@pytest.mark.parametrize("n", [64, 65, 87, 100, 127])
+@pytest.mark.parametrize("n", [3, 17, 40, 55, 63, 64, 65, 87, 100, 127])
 def test_diagonal_pivots_are_gauss_errc_on_normal_matrices(n):
     # damped normal matrices are symmetric positive definite: complete
     # pivoting finds its maximum on the diagonal, so the pivot order and
