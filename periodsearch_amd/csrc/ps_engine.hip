@@ -154,7 +154,7 @@ __host__ __device__ inline SlotMem slot_mem(int ma, int mfit, int npad) {
   m.a = o; o += kMaxMa;
   m.atry = o; o += kMaxMa;
   m.alast = o; o += kMaxMa;
-  // mixed forms with ordered points: sorted position -> point (int), and the
+  // ordered points (point_order): sorted position -> point (int), and the
   // points' brightness, sigma and weight/sigma^2 in sorted order
   m.perm = o; o += (npad + 1) / 2;
   m.pys = o; o += npad;
@@ -531,8 +531,9 @@ __host__ __device__ inline int neq_jn_doubles(int mfit, bool mix) {
 }
 
 template <int NT, bool MIX>
-__device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const double *J, double conw_r, double *wst, double *dave,
-                                      double *drat, const int *fidx, double *beta, double *chi_out, double *alpha) {
+__device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const double *J, const int *perm, double conw_r,
+                                                      double *wst, double *dave, double *drat, const int *fidx, double *beta,
+                                                      double *chi_out, double *alpha) {
 #pragma clang fp contract(off)
   using AccT = typename std::conditional<MIX, v4f, v4d>::type;
   using JnT = typename std::conditional<MIX, float, double>::type;
@@ -619,9 +620,10 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
           ymr = J[(size_t)ma * ld + q];
           double sg, wg, yo;
           if (q < P.n_points) {
-            sg = P.sig[q];
-            wg = P.w[q];
-            yo = P.y[q];
+            const int pt = perm ? perm[q] : q;  // Jacobian column q holds point pt
+            sg = P.sig[pt];
+            wg = P.w[pt];
+            yo = P.y[pt];
           } else {
             sg = 1 / conw_r;
             wg = 1;
@@ -1688,11 +1690,11 @@ __device__ __forceinline__ const DevProblem &pick_problem(const DevProblem &P0, 
 // registers of four waves (gauss_solve_block); otherwise 256 threads and the
 // LDS solver.
 constexpr int kWaveSolveMax = 63;
-// Point order of a fit (mixed forms): within each light curve, the points by
+// Point order of a fit: within each light curve, the points by
 // the body-frame longitude of the Earth-Sun bisector at the fit's starting
 // pole and period, in kOrderBins bins, ties in point order (a stable
 // counting sort, so the order is deterministic).  Points that see nearly
-// the same facets then share a model tile, and the model kernel's wave-wide
+// the same facets then share a model tile, and the model kernels' wave-wide
 // skip of facets that no point of the wave sees takes effect more often.
 // Only the order in which the normal equations sum the points changes.
 // Also writes the points' brightness, sigma and weight/sigma^2 in that
@@ -2465,6 +2467,251 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   }
 }
 
+// FP64 model with the facet stage and the contraction overlapped (the model
+// of period_search/bright.c:14-126, as model_tile).  One workgroup per (slot,
+// tile of kTilePts points), blockIdx.x = tile * nslots + slot; wave w owns
+// the tile's points 16w..16w+15.
+//   Lane l takes point l%16 and the facets f0 + 4k + l/16 (k = 0..7) of each
+//   32-facet chunk.  That is the A operand of v_mfma_f64_16x16x4f64's k-step
+//   k (A lane l: row l%16, k-group l/16), so the weights W[p][f] = Area_f
+//   s_pf stay in registers and never pass through LDS.  While the matrix
+//   cores contract chunk c (S[p][j] += sum_f W[p][f] Dsph[f][j], NB blocks of
+//   16 columns, B lane l: row k-group l/16, column l%16), the same wave
+//   evaluates chunk c+1's weights on the vector units.
+//   With ncoef = 16 NB + 1 (R1 = 1: lmax 6 and 8 give 49 and 81) the last
+//   column is one FP64 FMA per point-facet pair, not a 16-column block of
+//   which 15 would be padding.
+//   A group of four facets that no point of the wave sees (neither lit nor
+//   seen) has all its weights zero: the wave skips the rest of their facet
+//   stage and the k-step's MFMAs (wave-uniform branches; the skipped terms
+//   are exact zeros, so the sums are the same).  With the fit's point order
+//   (point_order) about 40% of the k-steps are skipped at the 5,000-point,
+//   800-facet shape, against 21% in input order.
+//   Dsph chunks are double-buffered in LDS (one barrier per chunk) and pass
+//   through registers one chunk ahead; the normals and areas of all facets
+//   are one LDS table.  D layout: lane l holds S[4r + l/16][l%16], r = 0..3.
+//   Tile positions are points in the fit's order (W.order) or input order;
+//   the Jacobian's columns follow the positions.
+struct F64Lds {
+  size_t x, blm, ftab, geom, dch, d1, total;
+};
+// LDS row stride of a staged Dsph chunk, in doubles: an odd multiple of 16,
+// so that the four rows one B-operand read touches fall on alternate halves
+// of the banks
+__host__ __device__ constexpr int f64_dsph_stride(int nb) { return (16 * nb) % 32 == 16 ? 16 * nb : 16 * nb + 16; }
+__host__ __device__ inline F64Lds f64_layout(int fpad, int nb, int r1) {
+  F64Lds L;
+  size_t o = 0;
+  L.x = o;
+  o += kMaxMa * sizeof(double);
+  L.blm = o;
+  o += 32 * sizeof(double);
+  L.ftab = o;  // [fpad] (nx, ny, nz, Area), zero past the last facet
+  o += (size_t)fpad * sizeof(double4);
+  L.geom = o;
+  o = align16(o + kTilePts * sizeof(PointGeom));
+  L.dch = o;  // [2][kFacetChunk][stride]
+  o += 2 * (size_t)kFacetChunk * f64_dsph_stride(nb) * sizeof(double);
+  L.d1 = o;  // [2][kFacetChunk][2]: column 16 nb and the table's zero beside it
+  o += r1 ? 2 * (size_t)kFacetChunk * 2 * sizeof(double) : 0;
+  L.total = o;
+  return L;
+}
+
+template <int NB, int R1, bool MULTI>
+__global__ void __launch_bounds__(kThreads, 2) model_f64_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
+                                                               DevWork W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
+  const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
+  if (!W.st[slot].eval) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
+  if (tile >= P.ntiles64) return;
+  constexpr int NC = 16 * NB, DS = f64_dsph_stride(NB);
+  constexpr int RP = 8 * (NB + R1);   // 16-byte pieces per row of the table (16 (NB + R1) doubles)
+  constexpr int NP = NC / 2 + R1;     // pieces staged per row: the NB blocks, then column NC's piece
+  constexpr int kPieces = kFacetChunk * NP;
+  constexpr int kStageIt = (kPieces + kThreads - 1) / kThreads;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, pi = lane & 15, kq = lane >> 4;
+  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  const int F = P.nfac, nch = P.fpad / kFacetChunk;
+  const F64Lds L = f64_layout(P.fpad, NB, R1);
+  double *x = reinterpret_cast<double *>(smb + L.x), *blm = reinterpret_cast<double *>(smb + L.blm);
+  double4 *ftab = reinterpret_cast<double4 *>(smb + L.ftab);
+  PointGeom *gst = reinterpret_cast<PointGeom *>(smb + L.geom);
+  double *dch = reinterpret_cast<double *>(smb + L.dch);
+  double *d1 = reinterpret_cast<double *>(smb + L.d1);
+  const v4u *__restrict__ dsq = reinterpret_cast<const v4u *>(P.dsphP);  // [fpad][16 (NB + R1)], zero-padded
+  v4u pre[kStageIt];
+  // (threads past the last piece repeat it: identical loads and stores)
+  auto load_chunk = [&](int c) __attribute__((always_inline)) {
+    static_for<0, kStageIt>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int it = decltype(ic)::value;
+      const int idx = min(tid + it * kThreads, kPieces - 1);
+      const int r = idx / NP, pc = idx - r * NP;
+      pre[it] = dsq[(size_t)(c * kFacetChunk + r) * RP + pc];
+    });
+  };
+  auto store_chunk = [&](int b) __attribute__((always_inline)) {
+    static_for<0, kStageIt>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int it = decltype(ic)::value;
+      const int idx = min(tid + it * kThreads, kPieces - 1);
+      const int r = idx / NP, pc = idx - r * NP;
+      double *dst = (R1 != 0 && pc == NC / 2) ? d1 + (b * kFacetChunk + r) * 2 : dch + (b * kFacetChunk + r) * DS + 2 * pc;
+      *reinterpret_cast<v4u *>(dst) = pre[it];
+    });
+  };
+
+  for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
+  for (int f = tid; f < P.fpad; f += kThreads) {
+    double4 t = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (f < F) {
+      t = P.nor4[f];
+      t.w = mem[M.area + f];
+    }
+    ftab[f] = t;
+  }
+  if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
+  load_chunk(0);
+  store_chunk(0);
+  if (nch > 1) load_chunk(1);
+  __syncthreads();
+  const int p0 = tile * kTilePts;
+  const int *__restrict__ perm = reinterpret_cast<const int *>(mem + M.perm);
+  if (tid < kTilePts) {
+    const int q = min(p0 + tid, P.n_points - 1);
+    PointGeom G;
+    point_geometry(P, W.order ? perm[q] : q, x, blm, G);
+    gst[tid] = G;
+  }
+  __syncthreads();
+  const int pw = 16 * wave + pi;  // the lane's position in the tile
+  const double e_0 = gst[pw].e[0], e_1 = gst[pw].e[1], e_2 = gst[pw].e[2];
+  const double s_0 = gst[pw].e0[0], s_1 = gst[pw].e0[1], s_2 = gst[pw].e0[2];
+  const double cl = exp(x[P.ma - 2]), cls = x[P.ma - 1];
+  double br = 0, v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0, s1 = 0;
+  v4d acc[NB];
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) acc[cb] = v4d{0, 0, 0, 0};
+  // weight of facet f at the lane's point, w = Area mu mu0 (cl + cls/(mu+mu0))
+  // if the facet is lit and seen (mu, mu0 > kTiny), else 0, and its terms of
+  // the partial sums: br, dbr/d(e) = sum Area mu0 (cl + cls mu0/(mu+mu0)^2) n,
+  // dbr/d(e0) likewise, and the sums of Area mu mu0 (cl-derivative) and of
+  // Area mu mu0/(mu+mu0) (cls-derivative).  A facet that is not visible
+  // takes zero area and a unit denominator, so every term is a finite zero.
+  // any: some lane of the wave sees its facet (wave-uniform).
+  auto weight = [&](int f, bool &any) __attribute__((always_inline)) {
+    const double4 t = ftab[f];
+    const double mu = e_0 * t.x + e_1 * t.y + e_2 * t.z;
+    const double mu0 = s_0 * t.x + s_1 * t.y + s_2 * t.z;
+    const bool vis = mu > kTiny && mu0 > kTiny;
+    any = __ballot(vis) != 0;
+    if (!any) return 0.0;
+    const double af = vis ? t.w : 0.0;
+    const double rd = recip(vis ? mu + mu0 : 1.0);
+    const double amu = af * mu, amu0 = af * mu0, am = amu * mu0;
+    const double w = am * (cl + cls * rd);
+    br += w;
+    scl += am;
+    scls += am * rd;
+    const double k = cls * (rd * rd);
+    const double ad = amu0 * (cl + k * mu0), ad0 = amu * (cl + k * mu);
+    v0 += ad * t.x;
+    v1 += ad * t.y;
+    v2 += ad * t.z;
+    u0 += ad0 * t.x;
+    u1 += ad0 * t.y;
+    u2 += ad0 * t.z;
+    return w;
+  };
+  // k-step k of chunk buffer b with the weights a of facets 4k + l/16
+  auto contract = [&](int b, int k, double a) __attribute__((always_inline)) {
+    const int kr = b * kFacetChunk + 4 * k + kq;
+    const double *db = dch + kr * DS + pi;
+    double bv[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; cb++) bv[cb] = db[16 * cb];
+#pragma unroll
+    for (int cb = 0; cb < NB; cb++) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[cb], acc[cb], 0, 0, 0);
+    if constexpr (R1 != 0) s1 = fma(a, d1[kr * 2], s1);
+  };
+  // wa: the weights of the chunk being contracted, ka: its k-steps with a
+  // visible facet (bit k)
+  double wa[8];
+  unsigned ka = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    bool any;
+    wa[k] = weight(4 * k + kq, any);
+    ka |= any ? 1u << k : 0u;
+  }
+  for (int c = 0; c + 1 < nch; c++) {
+    const int b = c & 1;
+    store_chunk(b ^ 1);  // chunk c+1; its buffer was last read before the previous barrier
+    if (c + 2 < nch) load_chunk(c + 2);
+    const int fn = (c + 1) * kFacetChunk + kq;
+    double wb[8];
+    unsigned kb = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (ka & (1u << k)) contract(b, k, wa[k]);
+      bool any;
+      wb[k] = weight(fn + 4 * k, any);
+      kb |= any ? 1u << k : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) wa[k] = wb[k];
+    ka = kb;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (ka & (1u << k)) contract((nch - 1) & 1, k, wa[k]);
+
+  // the four facet groups of each point: lanes l, l^16, l^32, l^48 add the
+  // same values, so every lane ends with the point's totals
+  auto quad = [](double v) __attribute__((always_inline)) {
+    v += __shfl_xor(v, 16, 64);
+    return v + __shfl_xor(v, 32, 64);
+  };
+  br = quad(br);
+  v0 = quad(v0);
+  v1 = quad(v1);
+  v2 = quad(v2);
+  u0 = quad(u0);
+  u1 = quad(u1);
+  u2 = quad(u2);
+  scl = quad(scl);
+  scls = quad(scls);
+  if constexpr (R1 != 0) s1 = quad(s1);
+  const int ld = P.npad;
+  double *J = mem + M.J;
+  const int p = p0 + pw;  // position
+  if (kq == 0 && p < P.n_points) {
+    const PointGeom G = gst[pw];
+    const double sc = G.scale;
+    const double V[3] = {v0, v1, v2}, U[3] = {u0, u1, u2};
+    double r[3];
+    rotation_derivs(P, W.order ? perm[p] : p, blm, G, V, U, r);
+    for (int k = 0; k < 3; k++) J[(size_t)(P.ncoef + k) * ld + p] = sc * r[k];
+    for (int i = 0; i < 3; i++) J[(size_t)(P.ncoef + 3 + i) * ld + p] = br * G.dphp[i];
+    J[(size_t)(P.ncoef + 6) * ld + p] = sc * scl * cl;
+    J[(size_t)(P.ncoef + 7) * ld + p] = sc * scls;
+    J[(size_t)P.ma * ld + p] = br * sc;
+    if (R1 != 0 && NC < P.ncoef) J[(size_t)NC * ld + p] = sc * s1;
+  }
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) {
+    const int c = 16 * cb + pi;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int pp = 16 * wave + 4 * r + kq;
+      const int q = p0 + pp;
+      if (q < P.n_points && c < P.ncoef) J[(size_t)c * ld + q] = gst[pp].scale * acc[cb][r];
+    }
+  }
+}
+
 // neq_kernel: one workgroup per slot.  Convexity rows, normal equations at
 // the trial parameters, then accept/reject and the driver bookkeeping.
 // LDS: staged rows (neq_jn_doubles), aux [3*kNeqChunk], dave [ma+1], x [ma],
@@ -2503,8 +2750,8 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
                               W.order ? mem + M.psw : P.swt, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave,
                               drat, fidx, btr, scal, atr);
     else
-      normal_equations_mfma<NT, MIX>(P, mem + M.J, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx,
-                                     btr, scal, atr);
+      normal_equations_mfma<NT, MIX>(P, mem + M.J, W.order ? reinterpret_cast<const int *>(mem + M.perm) : nullptr,
+                                     W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx, btr, scal, atr);
     __threadfence_block();
     __syncthreads();
   }
@@ -2755,6 +3002,38 @@ ModelFn pick_mfma(int ncoef) {
   }
 }
 
+// FP64 overlapped kernel: ncoef = 16 nb + 1 takes nb blocks and the
+// vector column, any other count whole blocks (zero-padded)
+template <bool MULTI>
+ModelFn pick_f64(int ncoef) {
+  if (ncoef % 16 == 1 && ncoef > 16) switch (ncoef / 16) {
+      case 1: return model_f64_kernel<1, 1, MULTI>;
+      case 2: return model_f64_kernel<2, 1, MULTI>;
+      case 3: return model_f64_kernel<3, 1, MULTI>;
+      case 4: return model_f64_kernel<4, 1, MULTI>;
+      case 5: return model_f64_kernel<5, 1, MULTI>;
+      case 6: return model_f64_kernel<6, 1, MULTI>;
+      case 7: return model_f64_kernel<7, 1, MULTI>;
+      default: return nullptr;
+    }
+  switch ((ncoef + 15) / 16) {
+    case 1: return model_f64_kernel<1, 0, MULTI>;
+    case 2: return model_f64_kernel<2, 0, MULTI>;
+    case 3: return model_f64_kernel<3, 0, MULTI>;
+    case 4: return model_f64_kernel<4, 0, MULTI>;
+    case 5: return model_f64_kernel<5, 0, MULTI>;
+    case 6: return model_f64_kernel<6, 0, MULTI>;
+    case 7: return model_f64_kernel<7, 0, MULTI>;
+    case 8: return model_f64_kernel<8, 0, MULTI>;
+    default: return nullptr;
+  }
+}
+size_t f64_lds(int nfac, int ncoef) {
+  const int fpad = (nfac + kFacetChunk - 1) / kFacetChunk * kFacetChunk;
+  const bool r1 = ncoef % 16 == 1 && ncoef > 16;
+  return f64_layout(fpad, r1 ? ncoef / 16 : (ncoef + 15) / 16, r1 ? 1 : 0).total;
+}
+
 template <int MODE, bool MULTI>
 ModelFn pick_mixed(int ncoef) {
   switch ((ncoef + 15) / 16) {
@@ -2933,6 +3212,8 @@ struct ps_engine {
   size_t lds_iter = 0;
   bool order_points = true;  // mixed forms: each fit's points ordered for the model kernel (PSAMD_ORDER=0: input order)
   bool diag_pivots = true;   // mixed forms: diagonal pivots in the four-wave solve (PSAMD_PIVOT=full: complete pivoting)
+  bool f64_barrier = false;  // FP64: the kernel whose facet stage and contraction alternate across barriers
+                             // (PSAMD_MODEL=mfma) instead of the overlapped one
   int order = 0;
   // slot memory
   SlotState *st = nullptr;
@@ -3002,8 +3283,12 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
                    : (e->precision == PS_PREC_BF16X3 ? 2 : (e->precision == PS_PREC_BF16X6 ? 3 : 0));
   ks.G = G;
   ks.use_mfma = mixed || !allow_valu;
-  if (!mixed)
-    ks.model = !ks.use_mfma ? pick_model<false>(G.ncp) : (multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef));
+  if (!mixed && !ks.use_mfma)
+    ks.model = pick_model<false>(G.ncp);
+  else if (!mixed && e->f64_barrier)
+    ks.model = multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef);
+  else if (!mixed)
+    ks.model = multi ? pick_f64<true>(ncoef) : pick_f64<false>(ncoef);
   else if (multi)
     ks.model = mode == 1 ? pick_mixed<1, true>(ncoef)
                : (mode == 2 ? pick_mixed<2, true>(ncoef) : (mode == 3 ? pick_mixed<3, true>(ncoef) : pick_mixed<0, true>(ncoef)));
@@ -3016,9 +3301,12 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   }
   const bool wave_solve = mfit <= kWaveSolveMax, reg_solve = mfit <= kBlockSolveMax;
   ks.lds_step = step_lds(reg_solve ? 0 : mfit, F);
-  ks.lds_model = mixed ? mixed_layout(F, nb, mode).total : (ks.use_mfma ? mfma_lds(F, ncoef) : model_lds(F));
+  ks.lds_model = mixed ? mixed_layout(F, nb, mode).total
+                       : (!ks.use_mfma ? model_lds(F) : (e->f64_barrier ? mfma_lds(F, ncoef) : f64_lds(F, ncoef)));
   const int neq_blocks = neq_ks_blocks(mfit, mixed, e->neq_staged);
-  ks.order = neq_blocks > 0 && e->order_points ? 1 : 0;
+  // ordered points: the mixed forms with the split normal equations, and
+  // FP64 with the overlapped model kernel
+  ks.order = e->order_points && (mixed ? neq_blocks > 0 : ks.use_mfma && !e->f64_barrier) ? 1 : 0;
   ks.lds_neq = neq_lds(mfit, mixed, neq_blocks);
   const size_t lds_max = 160 * 1024 - 1024;  // dynamic LDS per workgroup (static variables take the rest)
   if (ks.lds_step > lds_max || ks.lds_neq > lds_max || ks.lds_model > lds_max) {
@@ -3642,6 +3930,7 @@ int ps_create(int device, ps_engine **out) {
   if (const char *fu = getenv("PSAMD_FUSE")) e->fuse = atoi(fu) != 0;
   if (const char *po = getenv("PSAMD_ORDER")) e->order_points = atoi(po) != 0;
   if (const char *pv = getenv("PSAMD_PIVOT")) e->diag_pivots = strcmp(pv, "full") != 0;
+  if (const char *mk = getenv("PSAMD_MODEL")) e->f64_barrier = strcmp(mk, "mfma") == 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
     e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);

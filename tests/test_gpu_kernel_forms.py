@@ -12,7 +12,12 @@ other, at the 5,000-point shape (lmax 8, 800 facets, 87 free parameters):
 * the mixed forms' solve with diagonal pivots (default) against complete
   pivoting (PSAMD_PIVOT=full), in the four-wave and the one-wave solver: on
   damped normal matrices the two take the same pivots, so the fits are bit
-  for bit the same.
+  for bit the same;
+* the FP64 model kernels: the overlapped kernel (default; points in the
+  fit's order, zero facet blocks skipped), the same kernel in input order
+  (PSAMD_ORDER=0) and the kernel whose facet stage and contraction
+  alternate across barriers (PSAMD_MODEL=mfma): one evaluation agrees with
+  the oracle and between the kernels to the FP64 summation-order level.
 
 The switches are read when an engine is created, so each engine below is
 made with the environment it needs."""
@@ -139,3 +144,41 @@ def test_diagonal_and_complete_pivots_give_the_same_fits(big, tmp_path, workload
         eng.close()
     for k in FIT_RESULT_FIELDS:
         assert np.array_equal(res["diag"][k], res["full"][k]), k
+
+
+@pytest.mark.parametrize("workload", ["big", "wu"])
+def test_fp64_model_kernels_agree(big, tmp_path, workload):
+    # 81 coefficients (80 on the matrix cores + 1 vector column) and 49 (48 + 1)
+    import oracle
+    from periodsearch_amd import PS_PREC_FP64, Input, synth
+
+    if workload == "big":
+        inp, orc, kt = big
+    else:
+        path = str(tmp_path / "period_search_in")
+        ast = synth.write_workload("wu", path)
+        inp = Input(path)
+        kt = round((inp.freq_start - 1 / ast["period_h"]) / inp.freq_step) + 1
+        orc = oracle.Oracle(path)
+        orc.prepare(inp.conw_r0)
+    cgs = _starts(inp, orc, kt)
+    cg = np.array([c[1:] for c in cgs])
+    outs = {}
+    for name, env in (("overlapped", {}), ("input order", {"PSAMD_ORDER": "0"}), ("barrier", {"PSAMD_MODEL": "mfma"})):
+        eng = _engine(inp, PS_PREC_FP64, **env)
+        outs[name] = eng.eval_mrqcof(cg, inp.conw_r0)[:3]
+        eng.close()
+    for i, cg1 in enumerate(cgs):
+        A, b, c = orc.mrqcof(cg1)
+        for name, (alpha, beta, chi) in outs.items():
+            ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
+            eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+            ec = abs(chi[i] - c) / c
+            print(f"fp64 {workload} {name} start {i}: alpha {ea:.2e} beta {eb:.2e} chi2 {ec:.2e} against the oracle")
+            assert ea < 1e-12 and eb < 1e-12 and ec < 1e-12, (name, i, ea, eb, ec)
+        a0, b0, c0 = (outs["overlapped"][k][i] for k in range(3))
+        for name in ("input order", "barrier"):
+            a1, b1, c1 = (outs[name][k][i] for k in range(3))
+            assert np.abs(a1 - a0).max() / np.abs(a0).max() < 1e-13, name
+            assert np.abs(b1 - b0).max() / np.abs(b0).max() < 1e-13, name
+            assert abs(c1 - c0) / c0 < 1e-13, name
