@@ -433,7 +433,8 @@ def main() -> int:
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
             "traffic": round(tb * evals_launch) if tb else None,
-            "kernel": "model_mixed_kernel" if args.prec != "fp64" else "model_mfma_kernel",
+            "kernel": {"fp64": "model_f64_kernel", "fp32": "model_mixedw_kernel", "bf16": "model_mixedw_kernel"}.get(
+                args.prec, "model_mixed_kernel"),
             "ms_per_launch": round(ms_launch, 4), "evals_per_launch": round(evals_launch, 1),
             "launches": int(st["model_launches"]),
             "flop_per_eval": flop_eval,

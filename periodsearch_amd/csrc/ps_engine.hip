@@ -104,6 +104,9 @@ struct DevProblem {
   const float *dsphT32;
   const unsigned short *dsphT16, *dsphT16lo;  // bf16 of Dsph, and of Dsph - hi (split forms)
   const unsigned short *dsphT16lo2;           // bf16 of Dsph - hi - lo (three-part split form)
+  // dsphT16 with each 32-facet chunk permuted for the wave-owned-points
+  // kernel: chunk position 8g + j holds facet 4j + g
+  const unsigned short *dsphQ16;
   int fpad;
   const double4 *nor4;    // [F] (nx, ny, nz, 0): one 32-byte scalar load per facet
   const double *dsphP;    // [fpad][16*nb] Dsph zero-padded in rows and columns (FP64 matrix-core kernel)
@@ -2467,6 +2470,249 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
   }
 }
 
+// Mixed-precision model with wave-owned points, single-part forms
+// (PS_PREC_FP32: MODE 0, PS_PREC_BF16: MODE 1; the split forms keep
+// model_mixed_kernel, whose staging needs less LDS).  The model of
+// period_search/bright.c:14-126 as model_mixed_kernel.  One workgroup per
+// (slot, tile of kTilePts positions), blockIdx.x = tile * nslots + slot;
+// wave w owns the tile's positions 16w..16w+15, lane l the point l%16 and
+// the facet group g = l/16.
+//   mu and mu0 on the matrix cores: per half-chunk of 16 facets one
+//     v_mfma_f32_16x16x4f32 with A = the facets' normals (row m: facet
+//     4(m%4) + m/4 of the half-chunk; columns x, y, z and the area, which
+//     meets a zero row of B) and B = the directions e (or e0) of the wave's
+//     16 points: lane l receives mu for its point and the facets 4r + g,
+//     r = 0..3 (D lane l: rows 4(l/16)+r, column l%16).
+//   facet stage (FP32 VALU): for the lane's eight facets 16h + 4r + g the
+//     visibility test, and for a visible pair the weight
+//     W = Area mu mu0 (cl + cls/(mu+mu0)) and its derivative terms.  One
+//     step covers 16 points x 4 adjacent facets, and a step that no point of
+//     the wave sees is jumped over as a whole.
+//   contraction: the lane's eight weights are its A operand as they stand:
+//     FP32 k-step 4h + r takes facet 16h + 4r + g (the natural order of the
+//     Dsph^T table); bf16 k-slot 8g + j takes facet 4j + g (the order of the
+//     chunk-permuted bf16 tables).
+//   Dsph^T chunks are double-buffered in LDS (one barrier per chunk); the
+//   normals and areas are one LDS table, the lane's eight entries read
+//   together at the top of each chunk.
+//   On gfx950 the matrix and the vector instructions of a SIMD share its
+//   issue cycles (VALU and MFMA busy add up to 97% here), so the mu products
+//   pay off by their cycle count: 32 per 256 values against 48 for the three
+//   FMAs on the vector units.
+struct MixedWLds {
+  size_t x, blm, ftab, geom, dch, total;
+};
+__host__ __device__ inline MixedWLds mixedw_layout(int fpad, int nb, int mode) {
+  MixedWLds L;
+  size_t o = 0;
+  const bool bf = mode != 0;
+  const size_t es = bf ? 2 : 4, ws = kFacetChunk + (bf ? 8 : 4);
+  L.x = o;
+  o += kMaxMa * sizeof(double);
+  L.blm = o;
+  o += 32 * sizeof(double);
+  L.ftab = o;  // [fpad] (nx, ny, nz, Area) FP32, zero past the last facet
+  o = align16(o + (size_t)fpad * 16);
+  L.geom = o;
+  o = align16(o + kTilePts * sizeof(PointGeom));
+  L.dch = o;  // [2][16 nb][ws]
+  o = align16(o + 2 * 16 * (size_t)nb * ws * es);
+  L.total = o;
+  return L;
+}
+
+template <int NB, int MODE, bool MULTI>
+__global__ void __launch_bounds__(kThreads, 4) model_mixedw_kernel(DevProblem P0,
+                                                                             const DevProblem *__restrict__ probs,
+                                                                             DevWork W) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
+  const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
+  if (!W.st[slot].eval) return;
+  const DevProblem &P = pick_problem<MULTI>(P0, probs, W.st[slot].prob);
+  if (tile >= P.ntiles64) return;
+  static_assert(MODE == 0 || MODE == 1, "single-part forms");
+  constexpr bool BF = MODE != 0;
+  constexpr int NC = 16 * NB;
+  constexpr int WS = kFacetChunk + (BF ? 8 : 4);  // padded row, elements (80 / 144 bytes)
+  constexpr int BUF = NC * WS;                    // elements per buffer
+  using T = typename std::conditional<BF, unsigned short, float>::type;
+  constexpr int PER16 = 16 / (int)sizeof(T);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, pi = lane & 15, kg = lane >> 4;
+  const SlotMem M = slot_mem(P.ma, P.mfit, P.npad);
+  double *mem = W.mem + (size_t)slot * W.slot_doubles;
+  const int F = P.nfac, fpad = P.fpad, nch = fpad / kFacetChunk;
+  const MixedWLds L = mixedw_layout(fpad, NB, MODE);
+  double *x = reinterpret_cast<double *>(smb + L.x), *blm = reinterpret_cast<double *>(smb + L.blm);
+  float4 *ftab = reinterpret_cast<float4 *>(smb + L.ftab);
+  const float *ftf = reinterpret_cast<const float *>(smb + L.ftab);
+  PointGeom *gst = reinterpret_cast<PointGeom *>(smb + L.geom);
+  T *dch = reinterpret_cast<T *>(smb + L.dch);
+  const T *__restrict__ dsT = BF ? reinterpret_cast<const T *>(P.dsphQ16) : reinterpret_cast<const T *>(P.dsphT32);
+  constexpr int kPieces = kFacetChunk / PER16;  // 16-byte pieces per row
+  constexpr int kStageIt = (NC * kPieces + kThreads - 1) / kThreads;
+  v4u pre[kStageIt];
+  // (threads past the last piece repeat it: identical loads and stores)
+  auto load_chunk = [&](int fc) __attribute__((always_inline)) {
+    static_for<0, kStageIt>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int it = decltype(ic)::value;
+      const int idx = min(tid + it * kThreads, NC * kPieces - 1);
+      const int c = idx / kPieces, pc = idx - c * kPieces;
+      pre[it] = *reinterpret_cast<const v4u *>(dsT + (size_t)c * fpad + fc + pc * PER16);
+    });
+  };
+  auto store_chunk = [&](int b) __attribute__((always_inline)) {
+    T *d = dch + b * BUF;
+    static_for<0, kStageIt>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int it = decltype(ic)::value;
+      const int idx = min(tid + it * kThreads, NC * kPieces - 1);
+      const int c = idx / kPieces, pc = idx - c * kPieces;
+      *reinterpret_cast<v4u *>(d + c * WS + pc * PER16) = pre[it];
+    });
+  };
+
+  for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
+  for (int f = tid; f < fpad; f += kThreads)  // nor3f is zero-padded to fpad
+    ftab[f] = make_float4(P.nor3f[3 * f], P.nor3f[3 * f + 1], P.nor3f[3 * f + 2], f < F ? (float)mem[M.area + f] : 0.0f);
+  if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
+  load_chunk(0);
+  store_chunk(0);
+  if (nch > 1) load_chunk(kFacetChunk);
+  __syncthreads();
+  const int p0 = tile * kTilePts;
+  const int *__restrict__ perm = reinterpret_cast<const int *>(mem + M.perm);
+  if (tid < kTilePts) {
+    const int q = min(p0 + tid, P.n_points - 1);
+    PointGeom G;
+    point_geometry(P, W.order ? perm[q] : q, x, blm, G);
+    gst[tid] = G;
+  }
+  __syncthreads();
+  const int pw = 16 * wave + pi;  // the lane's position in the tile
+  // B operands of the mu and mu0 products: component g of the point's e and e0, zero for g = 3
+  const float be = kg == 0 ? (float)gst[pw].e[0] : (kg == 1 ? (float)gst[pw].e[1] : (kg == 2 ? (float)gst[pw].e[2] : 0.0f));
+  const float be0 = kg == 0 ? (float)gst[pw].e0[0] : (kg == 1 ? (float)gst[pw].e0[1] : (kg == 2 ? (float)gst[pw].e0[2] : 0.0f));
+  const double cld = exp(x[P.ma - 2]);
+  const float cl = (float)cld, cls = (float)x[P.ma - 1], tiny = (float)kTiny;
+  float v0 = 0, v1 = 0, v2 = 0, u0 = 0, u1 = 0, u2 = 0, scl = 0, scls = 0;
+  v4f acc[NB];
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) acc[cb] = v4f{0, 0, 0, 0};
+  const int fa = 4 * (pi & 3) + (pi >> 2);  // facet of A row pi within a half-chunk
+  const v4f z4 = {0, 0, 0, 0};
+  for (int c = 0; c < nch; c++) {
+    const int b = c & 1, f0 = c * kFacetChunk;
+    if (c + 1 < nch) {
+      store_chunk(b ^ 1);  // chunk c+1; its buffer was last read before the previous barrier
+      if (c + 2 < nch) load_chunk(f0 + 2 * kFacetChunk);
+    }
+    const float na0 = ftf[(f0 + fa) * 4 + kg], na1 = ftf[(f0 + 16 + fa) * 4 + kg];
+    // the lane's eight facets, read together (one wait for all of them)
+    float4 t8[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) t8[i] = ftab[f0 + 4 * i + kg];
+#pragma unroll
+    for (int i = 0; i < 8; i++) asm volatile("" ::"v"(t8[i].x), "v"(t8[i].y), "v"(t8[i].z), "v"(t8[i].w));
+    const v4f mua = __builtin_amdgcn_mfma_f32_16x16x4f32(na0, be, z4, 0, 0, 0);
+    const v4f mub = __builtin_amdgcn_mfma_f32_16x16x4f32(na1, be, z4, 0, 0, 0);
+    const v4f m0a = __builtin_amdgcn_mfma_f32_16x16x4f32(na0, be0, z4, 0, 0, 0);
+    const v4f m0b = __builtin_amdgcn_mfma_f32_16x16x4f32(na1, be0, z4, 0, 0, 0);
+    float wv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const float mu = i < 4 ? mua[i & 3] : mub[i & 3], mu0 = i < 4 ? m0a[i & 3] : m0b[i & 3];
+      float w = 0.0f;
+      if (mu > tiny && mu0 > tiny) {
+        // w = Area mu mu0 (cl + cls/(mu+mu0)) and its derivatives
+        // dw/dmu = Area mu0 (cl + cls mu0/(mu+mu0)^2), dw/dmu0 likewise; the
+        // brightness sum is cl*scl + cls*scls, formed after the loop
+        const float4 t = t8[i];
+        const float af = t.w;
+        const float rd = __builtin_amdgcn_rcpf(mu + mu0);
+        const float amu = af * mu, amu0 = af * mu0;
+        const float am = amu * mu0;
+        w = am * (cl + cls * rd);
+        scl += am;
+        scls += am * rd;
+        const float k = cls * (rd * rd);
+        const float ad = amu0 * (cl + k * mu0);
+        const float ad0 = amu * (cl + k * mu);
+        v0 += ad * t.x;
+        v1 += ad * t.y;
+        v2 += ad * t.z;
+        u0 += ad0 * t.x;
+        u1 += ad0 * t.y;
+        u2 += ad0 * t.z;
+      }
+      wv[i] = w;
+    }
+    const T *d = dch + b * BUF;
+    if constexpr (BF) {
+      // weight j is facet 4j + g: k-slot 8g + j of the permuted chunk
+      unsigned h[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) h[q] = bf16_pair(wv[2 * q], wv[2 * q + 1]);
+      const v8bf a = __builtin_bit_cast(v8bf, make_uint4(h[0], h[1], h[2], h[3]));
+#pragma unroll
+      for (int cb = 0; cb < NB; cb++) {
+        const v8bf bh = *reinterpret_cast<const v8bf *>(d + (16 * cb + pi) * WS + 8 * kg);
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[cb], 0, 0, 0);
+      }
+    } else {
+      // k-step s takes facet 4s + g
+#pragma unroll
+      for (int ks = 0; ks < kFacetChunk / 4; ks++) {
+#pragma unroll
+        for (int cb = 0; cb < NB; cb++) {
+          const float bv = d[(16 * cb + pi) * WS + 4 * ks + kg];
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wv[ks], bv, acc[cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // the four facet groups of each point: lanes l, l^16, l^32, l^48 add the
+  // same values, so every lane ends with the point's totals
+  auto quad = [](float v) __attribute__((always_inline)) {
+    v += __shfl_xor(v, 16, 64);
+    return v + __shfl_xor(v, 32, 64);
+  };
+  v0 = quad(v0);
+  v1 = quad(v1);
+  v2 = quad(v2);
+  u0 = quad(u0);
+  u1 = quad(u1);
+  u2 = quad(u2);
+  scl = quad(scl);
+  scls = quad(scls);
+  const int ld = P.npad;
+  double *J = mem + M.J;                     // row ma (ymod) in FP64
+  float *Jf = reinterpret_cast<float *>(J);  // rows 0..ma-1 in FP32
+  const int p = p0 + pw;                     // position
+  if (kg == 0 && p < P.n_points) {
+    const double br = cld * (double)scl + x[P.ma - 1] * (double)scls;  // sum of Area mu mu0 (cl + cls/(mu+mu0))
+    const PointGeom G = gst[pw];
+    const double sc = G.scale;
+    const double V[3] = {v0, v1, v2}, U[3] = {u0, u1, u2};
+    double r[3];
+    rotation_derivs(P, W.order ? perm[p] : p, blm, G, V, U, r);
+    for (int k = 0; k < 3; k++) Jf[(size_t)(P.ncoef + k) * ld + p] = (float)(sc * r[k]);
+    for (int i = 0; i < 3; i++) Jf[(size_t)(P.ncoef + 3 + i) * ld + p] = (float)(br * G.dphp[i]);
+    Jf[(size_t)(P.ncoef + 6) * ld + p] = (float)(sc * (double)scl * cld);
+    Jf[(size_t)(P.ncoef + 7) * ld + p] = (float)(sc * (double)scls);
+    J[(size_t)P.ma * ld + p] = br * sc;
+  }
+#pragma unroll
+  for (int cb = 0; cb < NB; cb++) {
+    const int c = 16 * cb + pi;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int pp = 16 * wave + 4 * kg + r;
+      const int q = p0 + pp;
+      if (q < P.n_points && c < P.ncoef) Jf[(size_t)c * ld + q] = (float)(gst[pp].scale * (double)acc[cb][r]);
+    }
+  }
+}
+
 // FP64 model with the facet stage and the contraction overlapped (the model
 // of period_search/bright.c:14-126, as model_tile).  One workgroup per (slot,
 // tile of kTilePts points), blockIdx.x = tile * nslots + slot; wave w owns
@@ -3035,6 +3281,21 @@ size_t f64_lds(int nfac, int ncoef) {
 }
 
 template <int MODE, bool MULTI>
+ModelFn pick_mixedw(int ncoef) {
+  switch ((ncoef + 15) / 16) {
+    case 1: return model_mixedw_kernel<1, MODE, MULTI>;
+    case 2: return model_mixedw_kernel<2, MODE, MULTI>;
+    case 3: return model_mixedw_kernel<3, MODE, MULTI>;
+    case 4: return model_mixedw_kernel<4, MODE, MULTI>;
+    case 5: return model_mixedw_kernel<5, MODE, MULTI>;
+    case 6: return model_mixedw_kernel<6, MODE, MULTI>;
+    case 7: return model_mixedw_kernel<7, MODE, MULTI>;
+    case 8: return model_mixedw_kernel<8, MODE, MULTI>;
+    default: return nullptr;
+  }
+}
+
+template <int MODE, bool MULTI>
 ModelFn pick_mixed(int ncoef) {
   switch ((ncoef + 15) / 16) {
     case 1: return model_mixed_kernel<1, MODE, MULTI>;
@@ -3214,6 +3475,8 @@ struct ps_engine {
   bool diag_pivots = true;   // mixed forms: diagonal pivots in the four-wave solve (PSAMD_PIVOT=full: complete pivoting)
   bool f64_barrier = false;  // FP64: the kernel whose facet stage and contraction alternate across barriers
                              // (PSAMD_MODEL=mfma) instead of the overlapped one
+  bool mixed_shared = false;  // FP32 and bf16 forms: the kernel whose waves share all 64 points
+                              // (PSAMD_MIXED=shared) instead of the one with wave-owned points
   int order = 0;
   // slot memory
   SlotState *st = nullptr;
@@ -3289,19 +3552,24 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
     ks.model = multi ? pick_mfma<true>(ncoef) : pick_mfma<false>(ncoef);
   else if (!mixed)
     ks.model = multi ? pick_f64<true>(ncoef) : pick_f64<false>(ncoef);
-  else if (multi)
+  else if ((e->mixed_shared || mode >= 2) && multi)
     ks.model = mode == 1 ? pick_mixed<1, true>(ncoef)
                : (mode == 2 ? pick_mixed<2, true>(ncoef) : (mode == 3 ? pick_mixed<3, true>(ncoef) : pick_mixed<0, true>(ncoef)));
-  else
+  else if (e->mixed_shared || mode >= 2)
     ks.model = mode == 1 ? pick_mixed<1, false>(ncoef)
                : (mode == 2 ? pick_mixed<2, false>(ncoef) : (mode == 3 ? pick_mixed<3, false>(ncoef) : pick_mixed<0, false>(ncoef)));
+  else if (multi)
+    ks.model = mode == 1 ? pick_mixedw<1, true>(ncoef) : pick_mixedw<0, true>(ncoef);
+  else
+    ks.model = mode == 1 ? pick_mixedw<1, false>(ncoef) : pick_mixedw<0, false>(ncoef);
   if (!ks.model) {
     e->err = "no model kernel for this coefficient count";
     return PS_ERR_LIMIT;
   }
   const bool wave_solve = mfit <= kWaveSolveMax, reg_solve = mfit <= kBlockSolveMax;
   ks.lds_step = step_lds(reg_solve ? 0 : mfit, F);
-  ks.lds_model = mixed ? mixed_layout(F, nb, mode).total
+  const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk;
+  ks.lds_model = mixed ? (e->mixed_shared || mode >= 2 ? mixed_layout(F, nb, mode).total : mixedw_layout(fpad, nb, mode).total)
                        : (!ks.use_mfma ? model_lds(F) : (e->f64_barrier ? mfma_lds(F, ncoef) : f64_lds(F, ncoef)));
   const int neq_blocks = neq_ks_blocks(mfit, mixed, e->neq_staged);
   // ordered points: the mixed forms with the split normal equations, and
@@ -3742,7 +4010,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk, ncT = 16 * nb;
   std::vector<float> dsphT32((size_t)ncT * fpad, 0.0f);
   std::vector<unsigned short> dsphT16((size_t)ncT * fpad, 0), dsphT16lo((size_t)ncT * fpad, 0),
-      dsphT16lo2((size_t)ncT * fpad, 0);
+      dsphT16lo2((size_t)ncT * fpad, 0), dsphQ16((size_t)ncT * fpad, 0);
   std::vector<double> dsphP((size_t)fpad * ncT, 0.0);
   auto to_bf16 = [](float v) {
     unsigned u;
@@ -3753,6 +4021,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
     nx[f] = p->nor[3 * f];
     ny[f] = p->nor[3 * f + 1];
     nz[f] = p->nor[3 * f + 2];
+    const int fi = f % kFacetChunk, fq = f - fi + 8 * (fi % 4) + fi / 4;  // position of f in the permuted chunk
     for (int c = 0; c < ncoef; c++) {
       const double v = p->dsph[(size_t)f * ncoef + c];
       dsph[(size_t)f * ncp + c] = v;
@@ -3770,6 +4039,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
       float lf;
       memcpy(&lf, &lb, 4);
       dsphT16lo2[(size_t)c * fpad + f] = to_bf16((vf - hf) - lf);
+      dsphQ16[(size_t)c * fpad + fq] = hi;
     }
     nor4[f] = make_double4(nx[f], ny[f], nz[f], 0.0);
     nor3f.push_back((float)nx[f]);
@@ -3822,6 +4092,7 @@ int upload_problem(ps_engine *e, const ps_problem *p, int nb_tab, DevProblem &D,
   UP(dsphT16.data(), dsphT16.size(), dsphT16);
   UP(dsphT16lo.data(), dsphT16lo.size(), dsphT16lo);
   UP(dsphT16lo2.data(), dsphT16lo2.size(), dsphT16lo2);
+  UP(dsphQ16.data(), dsphQ16.size(), dsphQ16);
   UP(cpleg.data(), cpleg.size(), cpleg);
   UP(cfc.data(), cfc.size(), cfc);
   UP(cfs.data(), cfs.size(), cfs);
@@ -3931,6 +4202,7 @@ int ps_create(int device, ps_engine **out) {
   if (const char *po = getenv("PSAMD_ORDER")) e->order_points = atoi(po) != 0;
   if (const char *pv = getenv("PSAMD_PIVOT")) e->diag_pivots = strcmp(pv, "full") != 0;
   if (const char *mk = getenv("PSAMD_MODEL")) e->f64_barrier = strcmp(mk, "mfma") == 0;
+  if (const char *mx = getenv("PSAMD_MIXED")) e->mixed_shared = strcmp(mx, "shared") == 0;
   if (const char *g = getenv("PSAMD_GROUPS")) {
     const int v = atoi(g);
     e->groups = v < 1 ? 1 : (v > kMaxGroups ? kMaxGroups : v);

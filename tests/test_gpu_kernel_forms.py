@@ -13,6 +13,10 @@ other, at the 5,000-point shape (lmax 8, 800 facets, 87 free parameters):
   pivoting (PSAMD_PIVOT=full), in the four-wave and the one-wave solver: on
   damped normal matrices the two take the same pivots, so the fits are bit
   for bit the same;
+* the FP32 and bf16 model kernels: wave-owned points (default; mu and mu0
+  on the matrix cores) against waves that share all 64 points
+  (PSAMD_MIXED=shared): both agree with the oracle, and with each other to
+  the FP32 rounding level (FP32) or within the bf16 form's bar (bf16);
 * the FP64 model kernels: the overlapped kernel (default; points in the
   fit's order, zero facet blocks skipped), the same kernel in input order
   (PSAMD_ORDER=0) and the kernel whose facet stage and contraction
@@ -144,6 +148,36 @@ def test_diagonal_and_complete_pivots_give_the_same_fits(big, tmp_path, workload
         eng.close()
     for k in FIT_RESULT_FIELDS:
         assert np.array_equal(res["diag"][k], res["full"][k]), k
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16"])
+def test_wave_owned_and_shared_point_kernels_agree(big, form):
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_FP32
+
+    inp, orc, kt = big
+    prec = {"fp32": PS_PREC_FP32, "bf16": PS_PREC_BF16}[form]
+    # bars against the oracle (alpha, beta) and between the kernels (alpha, beta, chi2)
+    bar = {"fp32": (1e-5, 1e-4, 2e-6, 2e-5, 1e-7), "bf16": (1e-2, 1.0, 1e-2, 1.0, 1e-6)}[form]
+    cgs = _starts(inp, orc, kt)
+    cg = np.array([c[1:] for c in cgs])
+    outs = {}
+    for name, env in (("wave", {}), ("shared", {"PSAMD_MIXED": "shared"})):
+        eng = _engine(inp, prec, **env)
+        outs[name] = eng.eval_mrqcof(cg, inp.conw_r0)[:3]
+        eng.close()
+    for i, cg1 in enumerate(cgs):
+        A, b, c = orc.mrqcof(cg1)
+        for name, (alpha, beta, chi) in outs.items():
+            ea = np.abs(alpha[i] - A).max() / np.abs(A).max()
+            eb = np.abs(beta[i] - b).max() / np.abs(b).max()
+            ec = abs(chi[i] - c) / c
+            print(f"{form} {name} start {i}: alpha {ea:.2e} beta {eb:.2e} chi2 {ec:.2e} against the oracle")
+            assert ea < bar[0] and eb < bar[1] and ec < 1e-6, (name, i, ea, eb, ec)
+        a0, b0, c0 = (outs["wave"][k][i] for k in range(3))
+        a1, b1, c1 = (outs["shared"][k][i] for k in range(3))
+        da, db, dc = np.abs(a1 - a0).max() / np.abs(a0).max(), np.abs(b1 - b0).max() / np.abs(b0).max(), abs(c1 - c0) / c0
+        print(f"{form} start {i}: wave against shared: alpha {da:.2e} beta {db:.2e} chi2 {dc:.2e}")
+        assert da < bar[2] and db < bar[3] and dc < bar[4], (i, da, db, dc)
 
 
 @pytest.mark.parametrize("workload", ["big", "wu"])
