@@ -522,6 +522,9 @@ __device__ void conv_rows(const DevProblem &P, const double *area, const double 
 // and alpha is summed by v_mfma_f32_16x16x4f32 (FP32 accumulation); the
 // light-curve sums, beta, chi2 and the residuals stay FP64.
 constexpr int kNeqChunk = 64;
+// entries of the normal-equation kernels' free-parameter table in LDS: the
+// staged kernel reads rows jf < 16*ceil(mfit/16) + 28 without a bound check
+constexpr int kFidxPad = (kMaxMfit + 15) / 16 * 16 + 32;
 
 // Doubles taken by the staged rows Jn (also the light-curve-sum and the
 // convexity-row scratch; ncoef <= mfit + 1).
@@ -603,10 +606,11 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
           lc_sums(J + (size_t)ca * ld, J + (size_t)cb * ld, two, ca, cb);
       }
       __syncthreads();
-      if (MIX) {  // dave[l]/ave, once per light curve
-        for (int col = tid; col < ma; col += kThreads) drat[col] = dave[col] / dave[ma];
-        __syncthreads();
-      }
+      // dave[l]/ave, once per light curve (the reference divides the product
+      // ymod*dave[l] by ave for every point; the quotient first differs from
+      // it by rounding only)
+      for (int col = tid; col < ma; col += kThreads) drat[col] = dave[col] / dave[ma];
+      __syncthreads();
     }
     for (int c0 = q0; c0 < q1; c0 += kNeqChunk) {
       const int cn = min(kNeqChunk, q1 - c0);
@@ -643,7 +647,6 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
           aux[pp * 3 + 1] = 0;
           aux[pp * 3 + 2] = 0;
         }
-        const double ave = dave[ma];
         // eight rows per batch: their loads are issued together (the row
         // indices come from LDS, so no load waits on another)
         for (int jf0 = wave; jf0 < MP; jf0 += 8 * kWaves) {
@@ -654,7 +657,7 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
 #pragma unroll
           for (int u = 0; u < 8; u++) {
             const int jf = jf0 + kWaves * u;
-            lv[u] = jf < mfit ? fidx[jf] : 0;
+            lv[u] = fidx[jf];  // zero past mfit: a valid row
             if constexpr (MIX)
               raw[u] = Jf[(size_t)lv[u] * ld + qc];
             else
@@ -672,10 +675,7 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
             double d = dv[u];
             if (inrel && jf < mfit && live) {
               const int l = lv[u];
-              if (MIX)
-                d = (l == 0) ? 0.0 : coef * (d - ymr * drat[l]);
-              else
-                d = (l == 0) ? 0.0 : coef * (d - ymr * dave[l] / ave);
+              d = (l == 0) ? 0.0 : coef * (d - ymr * drat[l]);
             }
             if (pp < cn4) Jn[pp * MS + jf] = (JnT)d;
           }
@@ -2978,15 +2978,15 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
   double *mem = W.mem + (size_t)slot * W.slot_doubles;
   double *wst = sm, *dave = wst + neq_scratch_doubles(mfit, MIX, KS) + 3 * kNeqChunk;
   double *x = dave + kMaxMa + 1, *scal = x + kMaxMa;  // scal[0] = chi2, [1..3] = chck
-  double *drat = scal + 4;                            // [ma] dave/ave (mixed forms)
-  int *fidx = reinterpret_cast<int *>(drat + kMaxMa); // [mfit] free parameter -> index into a[]
+  double *drat = scal + 4;                            // [ma] dave/ave of the current light curve
+  int *fidx = reinterpret_cast<int *>(drat + kMaxMa); // [kFidxPad] free parameter -> index into a[], zero past mfit
   const int eval = S.eval;
   const int sel = S.sel;
   double *atr = mem + (sel ? M.alpha0 : M.alpha1), *btr = mem + (sel ? M.beta0 : M.beta1);
   if (eval) {
     if (tid == 0) atomicAdd(W.done + 1, 1);
     for (int l = tid; l < ma; l += kThreads) x[l] = mem[M.atry + l];
-    for (int jf = tid; jf < mfit; jf += kThreads) fidx[jf] = P.fidx[jf];
+    for (int jf = tid; jf < kFidxPad; jf += kThreads) fidx[jf] = jf < mfit ? P.fidx[jf] : 0;
     conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1, wst);
     __threadfence_block();
     __syncthreads();
@@ -3330,7 +3330,7 @@ size_t model_lds(int nfac) {
   return ((size_t)kMaxMa + 32 + 2 * (size_t)nfac) * sizeof(double) + kThreads * sizeof(PointGeom);
 }
 size_t neq_lds(int mfit, bool mix, int ks) {
-  return ((size_t)neq_scratch_doubles(mfit, mix, ks) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8 + (kMaxMfit + 1) / 2) *
+  return ((size_t)neq_scratch_doubles(mfit, mix, ks) + 3 * kNeqChunk + 3 * kMaxMa + 1 + 8 + kFidxPad / 2) *
          sizeof(double);
 }
 

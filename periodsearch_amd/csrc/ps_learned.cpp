@@ -17,7 +17,9 @@
 //      (no two within halo of each other) and halo periods on either side of
 //      each are refitted from the reference's ten poles by the judge engine
 //      (any precision; FP64 makes the judged rows the plain FP64 rows), while
-//      the next block's proposed fits run on the screening engine;
+//      the next blocks' proposed fits run on the screening engine (the
+//      blocks' judge batches queue behind each other on one thread, so the
+//      screening never waits for the judge);
 //   4. the best judged row is walked to a local minimum of the judged chi2
 //      profile (its unjudged neighbours judged, a few at a time).
 // Judged rows replace the screened rows and are flagged; only they are
@@ -27,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <string>
 #include <thread>
 #include <vector>
@@ -345,19 +348,26 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
   long long fits_run = 0;
   for (int i = 0; i < N; i++) judged[i] = 0;
 
-  // judge work of the previous block, run on its own thread when the judge
-  // is another engine (so its fits overlap the next block's screening fits)
+  // judge batches.  When the judge is another engine each batch runs on a
+  // thread of its own that first joins the previous batch's thread: the
+  // batches run one after another on the judge engine, overlapped with the
+  // screening fits of the following blocks, and the main thread never waits
+  // for them until the screening is done.  Their rows are applied then, in
+  // order.
   struct Judge {
     std::vector<int> pos;  // positions in [0, N)
     std::vector<double> f;
     std::vector<int> pole;
     std::vector<ps_fit_out> out;
     int rc = PS_OK;
-  } running;
-  std::thread worker;
-  auto start_judge = [&](Judge &J) {
-    J.f.clear();
-    J.pole.clear();
+    std::thread th;
+  };
+  std::list<Judge> jobs;  // stable addresses for the threads
+  auto start_judge = [&](std::vector<int> pos) {
+    Judge *prev = jobs.empty() ? nullptr : &jobs.back();
+    jobs.emplace_back();
+    Judge &J = jobs.back();
+    J.pos = std::move(pos);
     for (int i : J.pos)
       for (int m = 0; m < npol; m++) {
         J.f.push_back(freq[i]);
@@ -368,22 +378,33 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
       J.rc = J.f.empty() ? PS_OK : ps_fit_batch(judge, J.f.data(), J.pole.data(), (int)J.f.size(), conw_r, J.out.data());
     };
     if (judge != e)
-      worker = std::thread(run);
+      J.th = std::thread([prev, run]() {
+        if (prev && prev->th.joinable()) prev->th.join();
+        run();
+      });
     else
       run();
   };
-  auto finish_judge = [&](Judge &J) -> int {
-    if (worker.joinable()) worker.join();
-    if (J.rc) return J.rc;
-    fits_run += (long long)J.out.size();
-    std::vector<ps_row> r(J.pos.size());
-    rows_from_fits(J.out.data(), (int)J.pos.size(), npol, p->n_points, r.data());
-    for (size_t k = 0; k < J.pos.size(); k++) {
-      rows[J.pos[k]] = r[k];
-      judged[J.pos[k]] = 1;
+  // waits for every started batch (the last thread joins the others) and
+  // applies their rows; the first failure is returned
+  auto finish_judges = [&]() -> int {
+    if (!jobs.empty() && jobs.back().th.joinable()) jobs.back().th.join();
+    int r = PS_OK;
+    for (Judge &J : jobs) {
+      if (J.rc) {
+        if (!r) r = J.rc;
+        continue;
+      }
+      fits_run += (long long)J.out.size();
+      std::vector<ps_row> rr(J.pos.size());
+      rows_from_fits(J.out.data(), (int)J.pos.size(), npol, p->n_points, rr.data());
+      for (size_t k = 0; k < J.pos.size(); k++) {
+        rows[J.pos[k]] = rr[k];
+        judged[J.pos[k]] = 1;
+      }
     }
-    J.pos.clear();
-    return PS_OK;
+    jobs.clear();
+    return r;
   };
 
   std::vector<float> feat, x, y;
@@ -443,12 +464,9 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
   rc = prepare(0, std::min(N, opt->block), cg);
   for (int b0 = 0; b0 < N && !rc; b0 += opt->block) {
     const int b1 = std::min(N, b0 + opt->block), nb = b1 - b0;
-    // 3. screening fits (the previous block's judge runs meanwhile)
+    // 3. screening fits (the earlier blocks' judge batches run meanwhile)
     out.assign((size_t)nb * per, ps_fit_out{});
-    rc = ps_fit_batch_from(e, cg.data(), nb * per, conw_r, out.data(), nullptr);
-    const int rc_j = finish_judge(running);
-    if (rc) break;
-    if ((rc = rc_j)) break;
+    if ((rc = ps_fit_batch_from(e, cg.data(), nb * per, conw_r, out.data(), nullptr))) break;
     fits_run += (long long)nb * per;
     brow.assign(nb, ps_row{});
     rows_from_fits(out.data(), nb, per, p->n_points, brow.data());
@@ -461,16 +479,15 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
     if (b1 < N && (rc = prepare(b1, std::min(N, b1 + opt->block) - b1, cg_next))) break;
     // 4. candidates of this block go to the judge
     std::vector<int> cand = pick_candidates(brow.data(), nb, opt->n_candidates, opt->halo);
-    running.pos.clear();
-    for (int i : cand) running.pos.push_back(b0 + i);
-    start_judge(running);
+    for (int &i : cand) i += b0;
+    start_judge(std::move(cand));
     cg.swap(cg_next);
   }
-  if (rc) {
-    if (worker.joinable()) worker.join();
-    return rc;
+  {
+    const int rc_j = finish_judges();
+    if (rc) return rc;
+    if ((rc = rc_j)) return rc;
   }
-  if ((rc = finish_judge(running))) return rc;
   // 5. the judged rows' chi2 over neighbouring trial periods is smooth near a
   //    minimum (neighbours converge to the same solution), so the best judged
   //    row is walked to a local minimum of the kept profile: while a
@@ -481,12 +498,12 @@ int ps_scan_learned(ps_engine *e, const ps_problem *p, const ps_proposal_net *ne
     for (int i = 0; i < N; i++)
       if (judged[i] && !std::isnan(rows[i].chisq) && (best < 0 || rows[i].chisq < rows[best].chisq)) best = i;
     if (best < 0) break;
-    running.pos.clear();
+    std::vector<int> nbr;
     for (int j : {best - 1, best + 1})
-      if (j >= 0 && j < N && !judged[j]) running.pos.push_back(j);
-    if (running.pos.empty()) break;
-    start_judge(running);
-    if ((rc = finish_judge(running))) return rc;
+      if (j >= 0 && j < N && !judged[j]) nbr.push_back(j);
+    if (nbr.empty()) break;
+    start_judge(std::move(nbr));
+    if ((rc = finish_judges())) return rc;
   }
   if (n_fits) *n_fits = fits_run;
   return PS_OK;
