@@ -60,7 +60,7 @@ HOST_CPU_SHARE = 16  # CPU threads the GPU box allots to one GPU
 # HBM bytes per model evaluation of the model kernel (copy-calibrated
 # FETCH_SIZE + WRITE_SIZE counter runs; profiles/), by workload and precision
 MODEL_HBM_BYTES_PER_EVAL = {
-    ("c2", "fp64"): 4186.0e3, ("c2", "bf16"): 2339.5e3,
+    ("c2", "fp64"): 4223.0e3, ("c2", "bf16"): 2333.0e3,
 }
 PREC_NAMES = ("fp64", "fp32", "bf16", "bf16x3", "bf16x6")
 DTYPE = {"fp64": "f64",

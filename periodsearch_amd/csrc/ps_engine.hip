@@ -2522,7 +2522,7 @@ __host__ __device__ inline MixedWLds mixedw_layout(int fpad, int nb, int mode) {
 }
 
 template <int NB, int MODE, bool MULTI>
-__global__ void __launch_bounds__(kThreads, 4) model_mixedw_kernel(DevProblem P0,
+__global__ void __launch_bounds__(kThreads, (MODE == 1 && NB <= 4) ? 5 : 4) model_mixedw_kernel(DevProblem P0,
                                                                              const DevProblem *__restrict__ probs,
                                                                              DevWork W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
