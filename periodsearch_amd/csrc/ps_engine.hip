@@ -2734,8 +2734,10 @@ __global__ void __launch_bounds__(kThreads, (MODE == 1 && NB <= 4) ? 5 : 4) mode
 //   (point_order) about 40% of the k-steps are skipped at the 5,000-point,
 //   800-facet shape, against 21% in input order.
 //   Dsph chunks are double-buffered in LDS (one barrier per chunk) and pass
-//   through registers one chunk ahead; the normals and areas of all facets
-//   are one LDS table.  D layout: lane l holds S[4r + l/16][l%16], r = 0..3.
+//   through registers one chunk ahead; the chunks' normals and areas go
+//   through a two-chunk LDS ring two chunks ahead of the contraction (the
+//   facet stage runs one ahead).  About 51 KB of LDS at NB = 5, so three
+//   workgroups share a CU.  D layout: lane l holds S[4r + l/16][l%16].
 //   Tile positions are points in the fit's order (W.order) or input order;
 //   the Jacobian's columns follow the positions.
 struct F64Lds {
@@ -2752,8 +2754,8 @@ __host__ __device__ inline F64Lds f64_layout(int fpad, int nb, int r1) {
   o += kMaxMa * sizeof(double);
   L.blm = o;
   o += 32 * sizeof(double);
-  L.ftab = o;  // [fpad] (nx, ny, nz, Area), zero past the last facet
-  o += (size_t)fpad * sizeof(double4);
+  L.ftab = o;  // [2][kFacetChunk] (nx, ny, nz, Area), zero past the last facet
+  o += 2 * (size_t)kFacetChunk * sizeof(double4);
   L.geom = o;
   o = align16(o + kTilePts * sizeof(PointGeom));
   L.dch = o;  // [2][kFacetChunk][stride]
@@ -2765,8 +2767,9 @@ __host__ __device__ inline F64Lds f64_layout(int fpad, int nb, int r1) {
 }
 
 template <int NB, int R1, bool MULTI>
-__global__ void __launch_bounds__(kThreads, 2) model_f64_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
-                                                               DevWork W) {
+__global__ void __launch_bounds__(kThreads, NB <= 5 ? 3 : 2) model_f64_kernel(DevProblem P0,
+                                                                             const DevProblem *__restrict__ probs,
+                                                                             DevWork W) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smb[];
   const int slot = blockIdx.x % W.nslots, tile = blockIdx.x / W.nslots;
   if (!W.st[slot].eval) return;
@@ -2808,15 +2811,17 @@ __global__ void __launch_bounds__(kThreads, 2) model_f64_kernel(DevProblem P0, c
     });
   };
 
-  for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
-  for (int f = tid; f < P.fpad; f += kThreads) {
+  // facet f's normal and area (zero past the last facet), for the ring
+  auto facet_entry = [&](int f) __attribute__((always_inline)) {
     double4 t = make_double4(0.0, 0.0, 0.0, 0.0);
     if (f < F) {
       t = P.nor4[f];
       t.w = mem[M.area + f];
     }
-    ftab[f] = t;
-  }
+    return t;
+  };
+  for (int l = tid; l < P.ma; l += kThreads) x[l] = mem[M.atry + l];
+  if (tid < 2 * kFacetChunk) ftab[tid] = facet_entry(tid);  // chunks 0 and 1
   if (tid == 0) eval_blmatrix(mem[M.atry + P.ncoef], mem[M.atry + P.ncoef + 1], blm);
   load_chunk(0);
   store_chunk(0);
@@ -2847,7 +2852,7 @@ __global__ void __launch_bounds__(kThreads, 2) model_f64_kernel(DevProblem P0, c
   // takes zero area and a unit denominator, so every term is a finite zero.
   // any: some lane of the wave sees its facet (wave-uniform).
   auto weight = [&](int f, bool &any) __attribute__((always_inline)) {
-    const double4 t = ftab[f];
+    const double4 t = ftab[f & (2 * kFacetChunk - 1)];  // ring slot of chunk f/32
     const double mu = e_0 * t.x + e_1 * t.y + e_2 * t.z;
     const double mu0 = s_0 * t.x + s_1 * t.y + s_2 * t.z;
     const bool vis = mu > kTiny && mu0 > kTiny;
@@ -2891,10 +2896,16 @@ __global__ void __launch_bounds__(kThreads, 2) model_f64_kernel(DevProblem P0, c
     wa[k] = weight(4 * k + kq, any);
     ka |= any ? 1u << k : 0u;
   }
+  __syncthreads();  // every wave has read chunk 0's ring slot before iteration 0 refills it
   for (int c = 0; c + 1 < nch; c++) {
     const int b = c & 1;
     store_chunk(b ^ 1);  // chunk c+1; its buffer was last read before the previous barrier
     if (c + 2 < nch) load_chunk(c + 2);
+    // chunk c+2's facet entries replace chunk c's, whose facet stage ran
+    // before the previous barrier; stored before this iteration's barrier
+    const bool fring = c + 2 < nch && tid < kFacetChunk;
+    double4 fnext;
+    if (fring) fnext = facet_entry((c + 2) * kFacetChunk + tid);
     const int fn = (c + 1) * kFacetChunk + kq;
     double wb[8];
     unsigned kb = 0;
@@ -2905,6 +2916,7 @@ __global__ void __launch_bounds__(kThreads, 2) model_f64_kernel(DevProblem P0, c
       wb[k] = weight(fn + 4 * k, any);
       kb |= any ? 1u << k : 0u;
     }
+    if (fring) ftab[b * kFacetChunk + tid] = fnext;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 8; k++) wa[k] = wb[k];
