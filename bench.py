@@ -60,18 +60,20 @@ HOST_CPU_SHARE = 16  # CPU threads the GPU box allots to one GPU
 # HBM bytes per model evaluation of the model kernel (copy-calibrated
 # FETCH_SIZE + WRITE_SIZE counter runs; profiles/), by workload and precision
 MODEL_HBM_BYTES_PER_EVAL = {
-    ("c2", "fp64"): 4223.0e3, ("c2", "bf16"): 2333.0e3,
+    ("c2", "fp64"): 4223.0e3, ("c2", "bf16"): 2333.0e3, ("c2", "bf16a"): 2333.0e3,  # bf16a: the bf16 model kernel
 }
-PREC_NAMES = ("fp64", "fp32", "bf16", "bf16x3", "bf16x6")
+PREC_NAMES = ("fp64", "fp32", "bf16", "bf16x3", "bf16x6", "bf16a")
 DTYPE = {"fp64": "f64",
          "fp32": "f32 contraction and alpha (MFMA, f32 accumulate), f64 geometry/chi2/solve",
          "bf16": "bf16 contraction (MFMA, f32 accumulate), f32 facet stage and alpha, f64 geometry/chi2/solve",
          "bf16x3": "split-bf16 contraction (3 MFMA products, f32 accumulate), f32 facet stage and alpha, "
                    "f64 geometry/chi2/solve",
          "bf16x6": "three-part split-bf16 contraction (6 MFMA products, f32 accumulate), f32 facet stage and alpha, "
-                   "f64 geometry/chi2/solve"}
+                   "f64 geometry/chi2/solve",
+         "bf16a": "bf16 contraction and alpha products (MFMA, f32 accumulate), f32 facet stage, f32/f64 beta, "
+                  "f64 geometry/chi2/solve"}
 FORM = {"fp64": "plain FP64", "fp32": "mixed FP32", "bf16": "mixed BF16", "bf16x3": "mixed split BF16",
-        "bf16x6": "mixed three-part split BF16"}
+        "bf16x6": "mixed three-part split BF16", "bf16a": "mixed BF16 with bf16 alpha (screening)"}
 WORKLOAD_TEXT = {
     "c2": "5000 points (20x150 dense + 2000 sparse, 2 apparitions over 420 d, 2% noise), lmax=mmax=8 (81 coef), "
           "nrows=10 (800 facets), 4-12 h (6720 trial periods), stop 50, 10 poles",
@@ -226,7 +228,7 @@ def main() -> int:
     ap.add_argument("--workload", choices=("c2", "wu", "batch64"), default="c2")
     ap.add_argument("--periods", type=int, default=0,
                     help="trial periods per step of the plain FP64 and mixed legs (0: 256 for c2, 2048 for wu)")
-    ap.add_argument("--prec", choices=PREC_NAMES, default="bf16",
+    ap.add_argument("--prec", choices=PREC_NAMES, default="bf16a",
                     help="precision of the mixed leg and of the learned form's screening fits (the judge is FP64)")
     ap.add_argument("--form", choices=("plain", "learned"), default="learned")
     ap.add_argument("--candidates", type=int, default=8, help="learned form: periods judged per block")
@@ -433,7 +435,8 @@ def main() -> int:
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
             "traffic": round(tb * evals_launch) if tb else None,
-            "kernel": {"fp64": "model_f64_kernel", "fp32": "model_mixedw_kernel", "bf16": "model_mixedw_kernel"}.get(
+            "kernel": {"fp64": "model_f64_kernel", "fp32": "model_mixedw_kernel", "bf16": "model_mixedw_kernel",
+                       "bf16a": "model_mixedw_kernel"}.get(
                 args.prec, "model_mixed_kernel"),
             "ms_per_launch": round(ms_launch, 4), "evals_per_launch": round(evals_launch, 1),
             "launches": int(st["model_launches"]),

@@ -53,12 +53,17 @@ extern "C" {
  * (PS_PREC_BF16; alpha stays FP32) or the contraction operands split into
  * bf16 high and low parts, three bf16 products (PS_PREC_BF16X3), or into
  * three bf16 parts, the six products down to third order (PS_PREC_BF16X6,
- * about 24 significant bits like FP32). */
+ * about 24 significant bits like FP32).  PS_PREC_BF16A is PS_PREC_BF16 with
+ * alpha's products also from bf16-rounded rows (beta and chi2 as in
+ * PS_PREC_BF16): a screening form, e.g. for ps_scan_learned, whose rows are
+ * not results; alpha differs from PS_PREC_BF16's by at most 2^-8 of its
+ * largest entry. */
 #define PS_PREC_FP64 0
 #define PS_PREC_FP32 1
 #define PS_PREC_BF16 2
 #define PS_PREC_BF16X3 3
 #define PS_PREC_BF16X6 4
+#define PS_PREC_BF16A 5
 
 typedef struct ps_engine ps_engine;
 

@@ -36,6 +36,7 @@ PS_PREC_FP32 = 1
 PS_PREC_BF16 = 2
 PS_PREC_BF16X3 = 3
 PS_PREC_BF16X6 = 4
+PS_PREC_BF16A = 5
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

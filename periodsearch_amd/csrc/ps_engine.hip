@@ -800,7 +800,21 @@ __device__ __forceinline__ float row_bcast(float v, int i) {
   return __builtin_bit_cast(float, r);
 }
 
-template <int NB>
+typedef short v4s __attribute__((ext_vector_type(4)));  // four bf16 (bit patterns) of one MFMA operand
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+// (a, b) rounded to bf16 (nearest even) and packed, a in the low half: one
+// v_cvt_pk_bf16_f32.
+__device__ inline unsigned bf16_pair(float a, float b) {
+  const v2bf h = __builtin_convertvector(v2f{a, b}, v2bf);
+  return __builtin_bit_cast(unsigned, h);
+}
+
+// ABF (PS_PREC_BF16A): alpha's products from the renormalised rows rounded
+// to bf16, one v_mfma_f32_16x16x16_bf16 per tile and 16-point group (A lane
+// l: row l%16, k = 4(l/16) + e, the lane's own four points) instead of four
+// v_mfma_f32_16x16x4f32; beta and chi2 keep their FP32/FP64 sums.
+template <int NB, bool ABF = false>
 __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const double *J, const double *__restrict__ yod,
                                                     const double *__restrict__ sgd, const double *__restrict__ swd,
                                                     double conw_r, double *ks, double *dave, double *drat,
@@ -953,6 +967,10 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
       float bp[NB];
 #pragma unroll
       for (int b = 0; b < NB; b++) bp[b] = 0.0f;
+      // ABF: the group's four points per block as bf16, each pair packed as
+      // soon as both points are formed
+      unsigned apk[ABF ? NB : 1][2], bpk[ABF ? NB : 1][2];
+      float ae[ABF ? NB : 1], be[ABF ? NB : 1];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const float cf = row_bcast(coef_f, i), cyi = row_bcast(cy_f, i), swi = row_bcast(sw_f, i),
@@ -967,13 +985,39 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
           bp[b] = bp[b] + jn * swdyi;
           bv[b] = jn;
           av[b] = jn * swi;
+          if constexpr (ABF) {
+            if (i & 1) {
+              apk[b][i >> 1] = bf16_pair(ae[b], av[b]);
+              bpk[b][i >> 1] = bf16_pair(be[b], bv[b]);
+            } else {
+              ae[b] = av[b];
+              be[b] = bv[b];
+            }
+          }
+        }
+        if constexpr (!ABF)
+          static_for<0, NB>([&](auto jbc) __attribute__((always_inline)) {
+            constexpr int jb = decltype(jbc)::value;
+            static_for<0, jb + 1>([&](auto kbc) __attribute__((always_inline)) {
+              constexpr int kb = decltype(kbc)::value;
+              constexpr int t = jb * (jb + 1) / 2 + kb;
+              acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[jb], bv[kb], acc[t], 0, 0, 0);
+            });
+          });
+      }
+      if constexpr (ABF) {
+        v4s ap[NB], bq[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          ap[b] = __builtin_bit_cast(v4s, make_uint2(apk[b][0], apk[b][1]));
+          bq[b] = __builtin_bit_cast(v4s, make_uint2(bpk[b][0], bpk[b][1]));
         }
         static_for<0, NB>([&](auto jbc) __attribute__((always_inline)) {
           constexpr int jb = decltype(jbc)::value;
           static_for<0, jb + 1>([&](auto kbc) __attribute__((always_inline)) {
             constexpr int kb = decltype(kbc)::value;
             constexpr int t = jb * (jb + 1) / 2 + kb;
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[jb], bv[kb], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap[jb], bq[kb], acc[t], 0, 0, 0);
           });
         });
       }
@@ -2174,7 +2218,7 @@ __global__ void __launch_bounds__(kThreads, NB <= 4 ? 3 : 2) model_mfma_kernel(D
   }
 }
 
-// Mixed-precision model (PS_PREC_FP32, PS_PREC_BF16, PS_PREC_BF16X3, PS_PREC_BF16X6).
+// Mixed-precision model (PS_PREC_FP32, PS_PREC_BF16, PS_PREC_BF16X3, PS_PREC_BF16X6; PS_PREC_BF16A as PS_PREC_BF16).
 // Per-point geometry (phase reduction, rotation matrices, phase function)
 // stays FP64 (point_geometry); the facet stage runs in FP32 and the
 // shape-derivative contraction S = W Dsph on the matrix cores with FP32
@@ -2222,14 +2266,6 @@ __host__ __device__ inline MixedLds mixed_layout(int nfac, int nb, int mode) {
   return L;
 }
 
-typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-typedef float v2f __attribute__((ext_vector_type(2)));
-// (a, b) rounded to bf16 (nearest even) and packed, a in the low half: one
-// v_cvt_pk_bf16_f32.
-__device__ inline unsigned bf16_pair(float a, float b) {
-  const v2bf h = __builtin_convertvector(v2f{a, b}, v2bf);
-  return __builtin_bit_cast(unsigned, h);
-}
 
 template <int NB, int MODE, bool MULTI>
 __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4) ? 5 : 4) model_mixed_kernel(DevProblem P0, const DevProblem *__restrict__ probs,
@@ -2471,7 +2507,7 @@ __global__ void __launch_bounds__(kThreads, ((MODE == 1 || MODE == 2) && NB <= 4
 }
 
 // Mixed-precision model with wave-owned points, single-part forms
-// (PS_PREC_FP32: MODE 0, PS_PREC_BF16: MODE 1; the split forms keep
+// (PS_PREC_FP32: MODE 0, PS_PREC_BF16 and PS_PREC_BF16A: MODE 1; the split forms keep
 // model_mixed_kernel, whose staging needs less LDS).  The model of
 // period_search/bright.c:14-126 as model_mixed_kernel.  One workgroup per
 // (slot, tile of kTilePts positions), blockIdx.x = tile * nslots + slot;
@@ -2977,7 +3013,7 @@ __global__ void __launch_bounds__(kThreads, NB <= 5 ? 3 : 2) model_f64_kernel(De
 #ifndef PS_NEQ_KS_WAVES
 #define PS_NEQ_KS_WAVES 2
 #endif
-template <int NT, bool MIX, bool MULTI, int KS>
+template <int NT, bool MIX, bool MULTI, int KS, bool ABF = false>
 __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem *__restrict__ probs, const DevWork &W) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x, slot = blockIdx.x;
@@ -3004,7 +3040,7 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
     __syncthreads();
     if constexpr (KS > 0)
       // the points' brightness, sigma and weight/sigma^2 in the Jacobian's column order
-      normal_equations_ks<KS>(P, mem + M.J, W.order ? mem + M.pys : P.y, W.order ? mem + M.psg : P.sig,
+      normal_equations_ks<KS, ABF>(P, mem + M.J, W.order ? mem + M.pys : P.y, W.order ? mem + M.psg : P.sig,
                               W.order ? mem + M.psw : P.swt, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave,
                               drat, fidx, btr, scal, atr);
     else
@@ -3098,9 +3134,9 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
   (void)atry;
 }
 
-template <int NT, bool MIX, bool MULTI, int KS = 0>
+template <int NT, bool MIX, bool MULTI, int KS = 0, bool ABF = false>
 __global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
-  neq_body<NT, MIX, MULTI, KS>(P0, probs, W);
+  neq_body<NT, MIX, MULTI, KS, ABF>(P0, probs, W);
 }
 
 // iter_kernel: one workgroup per slot, the end of one LM iteration and the
@@ -3111,9 +3147,9 @@ __global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_ke
 // parameters, where both parts run on four waves (the four-wave register
 // solver, BR rows per wave, or the LDS solver, BR = 0).  An LM iteration is
 // then two launches: this kernel and the model kernel.
-template <int NT, bool MIX, bool MULTI, int KS, int BR>
+template <int NT, bool MIX, bool MULTI, int KS, int BR, bool ABF = false>
 __global__ void __launch_bounds__(kThreads, 2) iter_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
-  neq_body<NT, MIX, MULTI, KS>(P0, probs, W);
+  neq_body<NT, MIX, MULTI, KS, ABF>(P0, probs, W);
   __syncthreads();
   step_body<MULTI, 0, BR>(P0, probs, W);
 }
@@ -3375,8 +3411,17 @@ inline int neq_ks_blocks(int mfit, bool mix, bool staged) {
 }
 
 template <bool MIX, bool MULTI>
-NeqFn pick_neq(int mfit, int ks) {
+NeqFn pick_neq(int mfit, int ks, bool abf) {
   if constexpr (MIX) {
+    if (abf) switch (ks) {
+        case 1: return neq_kernel<1, true, MULTI, 1, true>;
+        case 2: return neq_kernel<1, true, MULTI, 2, true>;
+        case 3: return neq_kernel<1, true, MULTI, 3, true>;
+        case 4: return neq_kernel<1, true, MULTI, 4, true>;
+        case 5: return neq_kernel<1, true, MULTI, 5, true>;
+        case 6: return neq_kernel<1, true, MULTI, 6, true>;
+        default: break;
+      }
     switch (ks) {
       case 1: return neq_kernel<1, true, MULTI, 1>;
       case 2: return neq_kernel<1, true, MULTI, 2>;
@@ -3398,8 +3443,11 @@ NeqFn pick_neq(int mfit, int ks) {
 // The fused iteration kernel for mfit > 63 free parameters (the solver's
 // rows per wave from mfit, the normal equations as pick_neq chooses them).
 template <bool MIX, bool MULTI>
-NeqFn pick_iter(int mfit, int ks) {
+NeqFn pick_iter(int mfit, int ks, bool abf) {
   if (mfit <= kWaveSolveMax || mfit > kMaxMfit) return nullptr;
+  if (MIX && abf && ks == 4 && mfit <= 64) return iter_kernel<1, MIX, MULTI, 4, 16, true>;
+  if (MIX && abf && ks == 5) return iter_kernel<1, MIX, MULTI, 5, 24, true>;
+  if (MIX && abf && ks == 6) return iter_kernel<1, MIX, MULTI, 6, 24, true>;
   if (MIX && ks == 4 && mfit <= 64) return iter_kernel<1, MIX, MULTI, 4, 16>;
   if (MIX && ks == 5) return iter_kernel<1, MIX, MULTI, 5, 24>;
   if (MIX && ks == 6) return iter_kernel<1, MIX, MULTI, 6, 24>;
@@ -3554,7 +3602,7 @@ void problem_dims(const ps_problem &p, int &ncoef, int &mfit) {
 int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool allow_valu, KernelSet &ks) {
   const int F = G.nfac, ncoef = G.ncoef, mfit = G.mfit;
   const bool mixed = e->precision != PS_PREC_FP64;
-  const int mode = e->precision == PS_PREC_BF16 ? 1
+  const int mode = (e->precision == PS_PREC_BF16 || e->precision == PS_PREC_BF16A) ? 1
                    : (e->precision == PS_PREC_BF16X3 ? 2 : (e->precision == PS_PREC_BF16X6 ? 3 : 0));
   ks.G = G;
   ks.use_mfma = mixed || !allow_valu;
@@ -3594,15 +3642,16 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
     return PS_ERR_LIMIT;
   }
   const bool diag = mixed && e->diag_pivots;  // FP64 keeps gauss_errc's complete pivoting
+  const bool abf = e->precision == PS_PREC_BF16A && neq_blocks > 0;  // alpha's products in bf16
   ks.step = multi ? pick_step<true>(mfit, diag) : pick_step<false>(mfit, diag);
   ks.step_threads = wave_solve ? 64 : kThreads;
   if (e->fuse) {
-    ks.iter = mixed ? (multi ? pick_iter<true, true>(mfit, neq_blocks) : pick_iter<true, false>(mfit, neq_blocks))
-                    : (multi ? pick_iter<false, true>(mfit, 0) : pick_iter<false, false>(mfit, 0));
+    ks.iter = mixed ? (multi ? pick_iter<true, true>(mfit, neq_blocks, abf) : pick_iter<true, false>(mfit, neq_blocks, abf))
+                    : (multi ? pick_iter<false, true>(mfit, 0, false) : pick_iter<false, false>(mfit, 0, false));
     ks.lds_iter = std::max(ks.lds_step, ks.lds_neq);
   }
-  ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit, neq_blocks) : pick_neq<true, false>(mfit, neq_blocks))
-                 : (multi ? pick_neq<false, true>(mfit, 0) : pick_neq<false, false>(mfit, 0));
+  ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit, neq_blocks, abf) : pick_neq<true, false>(mfit, neq_blocks, abf))
+                 : (multi ? pick_neq<false, true>(mfit, 0, false) : pick_neq<false, false>(mfit, 0, false));
   if (!ks.neq) {
     e->err = "too many free parameters";
     return PS_ERR_LIMIT;
@@ -4251,7 +4300,7 @@ const char *ps_last_error(const ps_engine *e) { return e ? e->err.c_str() : "nul
 
 int ps_set_precision(ps_engine *e, int precision) {
   if (!e) return PS_ERR_ARG;
-  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16X6) {
+  if (precision < PS_PREC_FP64 || precision > PS_PREC_BF16A) {
     e->err = "unknown precision";
     return PS_ERR_ARG;
   }

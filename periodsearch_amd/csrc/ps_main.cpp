@@ -3,7 +3,7 @@
 // that runs the fit loop on the GPU through libpsamd.
 //
 //   period_search_amd [--device N] [--in F] [--out F] [--state F] [--batch P]
-//                     [--round-pole] [--prec fp64|fp32|bf16|bf16x3|bf16x6]
+//                     [--round-pole] [--prec fp64|fp32|bf16|bf16x3|bf16x6|bf16a]
 //                     [--state-format base|cuda|oc] [--clamp-min] [--progress]
 //                     [--learned [--net F] [--block B] [--candidates K] [--halo H]
 //                      [--refine R] [--mirror] [--judged-out F]]
@@ -50,6 +50,7 @@ int parse_prec(const std::string &s) {
   if (s == "bf16") return PS_PREC_BF16;
   if (s == "bf16x3") return PS_PREC_BF16X3;
   if (s == "bf16x6") return PS_PREC_BF16X6;
+  if (s == "bf16a") return PS_PREC_BF16A;
   return -1;
 }
 
@@ -124,7 +125,7 @@ int main(int argc, char **argv) {
       const std::string v = next();
       prec = parse_prec(v);
       if (prec < 0) {
-        fprintf(stderr, "unknown precision '%s' (fp64, fp32, bf16, bf16x3, bf16x6)\n", v.c_str());
+        fprintf(stderr, "unknown precision '%s' (fp64, fp32, bf16, bf16x3, bf16x6, bf16a)\n", v.c_str());
         return 2;
       }
     } else if (a == "--state-format") {

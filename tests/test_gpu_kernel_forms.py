@@ -13,6 +13,9 @@ other, at the 5,000-point shape (lmax 8, 800 facets, 87 free parameters):
   pivoting (PSAMD_PIVOT=full), in the four-wave and the one-wave solver: on
   damped normal matrices the two take the same pivots, so the fits are bit
   for bit the same;
+* PS_PREC_BF16A against PS_PREC_BF16: beta and chi2 bit for bit, alpha
+  within the bound that bf16 rounding of both operands of every product
+  allows (2^-8 of sqrt(alpha_jj alpha_kk) per entry, by Cauchy-Schwarz);
 * the FP32 and bf16 model kernels: wave-owned points (default; mu and mu0
   on the matrix cores) against waves that share all 64 points
   (PSAMD_MIXED=shared): both agree with the oracle, and with each other to
@@ -107,12 +110,12 @@ def test_split_and_staged_normal_equations_agree(big, form):
             assert abs(c1 - c0) / c0 < 1e-7, name
 
 
-@pytest.mark.parametrize("form", ["fp64", "bf16"])
+@pytest.mark.parametrize("form", ["fp64", "bf16", "bf16a"])
 def test_fused_iteration_kernel_equals_two_launches(big, form):
-    from periodsearch_amd import PS_PREC_BF16, PS_PREC_FP64
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16A, PS_PREC_FP64
 
     inp, orc, kt = big
-    prec = {"fp64": PS_PREC_FP64, "bf16": PS_PREC_BF16}[form]
+    prec = {"fp64": PS_PREC_FP64, "bf16": PS_PREC_BF16, "bf16a": PS_PREC_BF16A}[form]
     freq = [inp.freq_start - (kt - 1 + d) * inp.freq_step for d in (0, 1)]
     freqs = [f for f in freq for _ in range(10)]
     poles = list(range(10)) * 2
@@ -148,6 +151,32 @@ def test_diagonal_and_complete_pivots_give_the_same_fits(big, tmp_path, workload
         eng.close()
     for k in FIT_RESULT_FIELDS:
         assert np.array_equal(res["diag"][k], res["full"][k]), k
+
+
+def test_bf16a_alpha_within_the_bf16_rounding_bound(big):
+    # each product w*jn_j*jn_k takes both operands rounded to bf16 (relative
+    # error at most 2^-9 each), so it moves by at most (2^-8 + 2^-18) of its
+    # size, and sum_p w |jn_j jn_k| <= sqrt(alpha_jj alpha_kk); the FP32 sums
+    # of the two forms add their own ~1e-7
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16A
+
+    inp, orc, kt = big
+    cgs = _starts(inp, orc, kt)
+    cg = np.array([c[1:] for c in cgs])
+    outs = {}
+    for prec in (PS_PREC_BF16, PS_PREC_BF16A):
+        eng = _engine(inp, prec)
+        outs[prec] = eng.eval_mrqcof(cg, inp.conw_r0)[:3]
+        eng.close()
+    (a0, b0, c0), (a1, b1, c1) = outs[PS_PREC_BF16], outs[PS_PREC_BF16A]
+    assert np.array_equal(b0, b1) and np.array_equal(c0, c1)
+    for i in range(len(cgs)):
+        d = np.sqrt(np.abs(np.diag(a0[i])))
+        bound = (2.0 ** -8 + 2.0 ** -18) * np.outer(d, d) + 1e-6 * np.abs(a0[i]).max()
+        err = np.abs(a1[i] - a0[i])
+        print(f"start {i}: alpha moved by {err.max() / np.abs(a0[i]).max():.2e} of its largest entry; "
+              f"largest share of the bound {(err / bound).max():.3f}")
+        assert np.all(err <= bound), i
 
 
 @pytest.mark.parametrize("form", ["fp32", "bf16"])

@@ -126,23 +126,24 @@ def test_learned_scan_keeps_the_plain_best_period_with_fewer_fits():
     prop.close()
 
 
-@pytest.mark.parametrize("prec_name", ["bf16", "bf16x6"])
+@pytest.mark.parametrize("prec_name", ["bf16", "bf16x6", "bf16a"])
 def test_learned_scan_at_configuration_4(prec_name):
     """The learned form on 16 held-out asteroids (seeds 5000-5015) at the
     mixed-precision configuration's shape (5,000 points, lmax = mmax = 8, 800
-    facets), 48 trial periods around each true period: screening fits in bf16
-    (the bench's default form) or the three-part split bf16, the 4 best
+    facets), 48 trial periods around each true period: screening fits in bf16,
+    in the three-part split bf16, or in bf16 with bf16 alpha products (the
+    bench's default screening form, PS_PREC_BF16A), the 4 best
     candidates and 2 neighbours each judged by plain FP64 ten-pole fits.  The
     best judged row is the plain FP64 scan's best row on every asteroid (same
     trial period; the judged rows are the plain FP64 fits themselves, so the
     same chi2).  How the screened rows compare with the plain rows of their
     periods is printed and bounded."""
-    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16X6
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16A, PS_PREC_BF16X6
     from periodsearch_amd import proposal as pr
 
     c2 = dict(lmax=8, mmax=8, nrows=10, n_dense=20, dense_pts=150, n_sparse=2000, noise=0.02)
     prop = pr.Proposal.load()
-    prec = {"bf16": PS_PREC_BF16, "bf16x6": PS_PREC_BF16X6}[prec_name]
+    prec = {"bf16": PS_PREC_BF16, "bf16x6": PS_PREC_BF16X6, "bf16a": PS_PREC_BF16A}[prec_name]
     ev = pr.evaluate(prop, n_asteroids=16, seed0=5000, window=48, cfg_over=c2, prec=prec)
     print({k: v for k, v in ev.items() if k != "runs"})
     for r in ev["runs"]:
@@ -155,7 +156,7 @@ def test_learned_scan_at_configuration_4(prec_name):
     prop.close()
 
 
-@pytest.mark.parametrize("prec_name", ["bf16", "bf16x6", "fp32"])
+@pytest.mark.parametrize("prec_name", ["bf16", "bf16x6", "fp32", "bf16a"])
 def test_learned_scan_in_bench_blocks_keeps_the_plain_best_period(tmp_path, prec_name):
     """The learned configuration on the work-unit asteroid (1080 points, lmax
     6, 512 facets): one block of 4096 trial periods around the true period,
@@ -168,14 +169,15 @@ def test_learned_scan_in_bench_blocks_keeps_the_plain_best_period(tmp_path, prec
     bounded."""
     import numpy as np
 
-    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16X6, PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
+    from periodsearch_amd import PS_PREC_BF16, PS_PREC_BF16A, PS_PREC_BF16X6, PS_PREC_FP32, PS_PREC_FP64, Engine, Input, synth
     from periodsearch_amd import proposal as pr
 
     path = str(tmp_path / "period_search_in")
     synth.write_workload("wu", path)
     inp = Input(path)
     eng = Engine(0)
-    eng.set_precision({"fp32": PS_PREC_FP32, "bf16": PS_PREC_BF16, "bf16x6": PS_PREC_BF16X6}[prec_name])
+    eng.set_precision({"fp32": PS_PREC_FP32, "bf16": PS_PREC_BF16, "bf16x6": PS_PREC_BF16X6,
+                       "bf16a": PS_PREC_BF16A}[prec_name])
     eng.set_slots(4096)
     eng.set_problem(inp.problem, keep=inp)
     e64 = Engine(0)
