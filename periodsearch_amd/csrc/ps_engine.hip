@@ -3135,7 +3135,7 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
 }
 
 template <int NT, bool MIX, bool MULTI, int KS = 0, bool ABF = false>
-__global__ void __launch_bounds__(kThreads, KS > 0 ? PS_NEQ_KS_WAVES : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+__global__ void __launch_bounds__(kThreads, KS > 0 ? ((ABF && KS <= 4) ? 3 : PS_NEQ_KS_WAVES) : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
   neq_body<NT, MIX, MULTI, KS, ABF>(P0, probs, W);
 }
 
