@@ -612,6 +612,26 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
       for (int col = tid; col < ma; col += kThreads) drat[col] = dave[col] / dave[ma];
       __syncthreads();
     }
+    // the rows of a chunk (this lane's point, kRounds batches of eight rows
+    // per wave) are loaded before the chunk starts: the light curve's first
+    // chunk here, each next one while the current one is being summed
+    constexpr int kRounds = NT <= 1 ? 1 : (NT <= 3 ? 2 : (NT <= 6 ? 3 : 4));  // >= 16*nb / (8*kWaves)
+    JnT pre[kRounds][8];
+    auto load_rows = [&](int cs) __attribute__((always_inline)) {
+      const int qn = cs + lane;
+      const int qc = qn < q1 ? qn : q0;  // a valid point for idle lanes
+#pragma unroll
+      for (int r = 0; r < kRounds; r++)
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int row = fidx[wave + 8 * kWaves * r + kWaves * u];  // zero past mfit: a valid row
+          if constexpr (MIX)
+            pre[r][u] = Jf[(size_t)row * ld + qc];
+          else
+            pre[r][u] = J[(size_t)row * ld + qc];
+        }
+    };
+    load_rows(q0);
     for (int c0 = q0; c0 < q1; c0 += kNeqChunk) {
       const int cn = min(kNeqChunk, q1 - c0);
       const int cn4 = (cn + 3) & ~3;
@@ -647,26 +667,18 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
           aux[pp * 3 + 1] = 0;
           aux[pp * 3 + 2] = 0;
         }
-        // eight rows per batch: their loads are issued together (the row
-        // indices come from LDS, so no load waits on another)
-        for (int jf0 = wave; jf0 < MP; jf0 += 8 * kWaves) {
+        // eight rows per batch, from the rows loaded before the chunk
+#pragma unroll
+        for (int r = 0; r < kRounds; r++) {
+          const int jf0 = wave + 8 * kWaves * r;
+          if (jf0 >= MP) break;
           double dv[8];
           int lv[8];
-          JnT raw[8];
-          const int qc = live ? q : q0;  // loads without branches: a valid point for idle lanes
 #pragma unroll
           for (int u = 0; u < 8; u++) {
             const int jf = jf0 + kWaves * u;
-            lv[u] = fidx[jf];  // zero past mfit: a valid row
-            if constexpr (MIX)
-              raw[u] = Jf[(size_t)lv[u] * ld + qc];
-            else
-              raw[u] = J[(size_t)lv[u] * ld + qc];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; u++) {
-            const int jf = jf0 + kWaves * u;
-            dv[u] = (jf < mfit && live) ? (double)raw[u] : 0.0;
+            lv[u] = fidx[jf];
+            dv[u] = (jf < mfit && live) ? (double)pre[r][u] : 0.0;
           }
 #pragma unroll
           for (int u = 0; u < 8; u++) {
@@ -681,6 +693,7 @@ __device__ __forceinline__ void normal_equations_mfma(const DevProblem &P, const
           }
         }
       }
+      if (c0 + kNeqChunk < q1) load_rows(c0 + kNeqChunk);  // in flight during this chunk's sums
       __syncthreads();
       for (int kk = 0; kk < cn4 / 4; kk++) {
         const int row = 4 * kk + (lane >> 4);
