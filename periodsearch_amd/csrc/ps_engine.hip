@@ -1091,12 +1091,258 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
   });
 }
 
+// FP64 normal equations with the points split over the waves
+// (mrqcof.c:42-139): normal_equations_ks's layout and order of work in FP64,
+// one v_mfma_f64_16x16x4f64 per (lower-triangle tile, step) summing four
+// distinct points, no staging through LDS.  Lane (c, q) reads the FP64 rows
+// of free parameters 16*b + c at points 4q..4q+3 of a 16-point group as one
+// 32-byte load per block; the next group's loads are issued before the
+// current group is computed.  Each per-point value is formed in FP64 with
+// the reference's operations, as the staged kernel forms them:
+//   coef = sig*np/ave; Jn = coef*(dyda - ymod*(dave/ave)); dy = y - coef*ymod;
+//   wt = Jn*sig2i*w; alpha += wt*Jn'; beta += dy*wt; chi2 += dy*dy*sig2i*w,
+// by lane 16q + s for point 4q + s, and reaches the other lanes of its row
+// by DPP row broadcasts.  Only the order of the sums differs from the
+// reference: per lane over its points, the lanes of a column and the waves
+// in a fixed order, so the result does not depend on timing.
+// perm: the fit's point order (Jacobian column q holds point perm[q]), or
+// null.  LDS (doubles): [4][16*NB+1] column sums, [4][16*NB] beta, [4] chi2,
+// then 4 tiles x 4 waves x 256 FP64 partial tiles.
+__host__ __device__ constexpr int neq_ks64_doubles(int nb) { return 4 * (16 * nb + 1) + 4 * 16 * nb + 4 + 4096; }
+
+// v broadcast from lane i of its row of 16 lanes (two DPP row_newbcast)
+__device__ __forceinline__ double row_bcast_d(double v, int i) {
+  const float lo = __builtin_bit_cast(float, __double2loint(v)), hi = __builtin_bit_cast(float, __double2hiint(v));
+  return __hiloint2double(__builtin_bit_cast(int, row_bcast(hi, i)), __builtin_bit_cast(int, row_bcast(lo, i)));
+}
+
+template <int NB>
+__device__ __forceinline__ void normal_equations_ks64(const DevProblem &P, const double *J, const int *perm,
+                                                      double conw_r, double *ks, double *dave, double *drat,
+                                                      const int *fidx, double *beta, double *chi_out, double *alpha) {
+#pragma clang fp contract(off)
+  constexpr int NT = NB * (NB + 1) / 2, NC = 16 * NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c16 = lane & 15, kq = lane >> 4, i4 = c16 & 3;
+  const int ld = P.npad, mfit = P.mfit, ma = P.ma, npts = P.n_points;
+  const double *ym = J + (size_t)ma * ld;  // model row
+  double *csum = ks, *bsum = ks + 4 * (NC + 1), *csq = bsum + 4 * NC;
+  double *tred = csq + 4;
+  const double sconv = 1 / conw_r;  // convexity rows: sigma (weight 1, brightness 0)
+  int roff[NB];
+  bool l0[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int jf = 16 * b + c16;
+    const int l = jf < mfit ? fidx[jf] : 0;
+    roff[b] = l * ld + 4 * kq;
+    l0[b] = l == 0;
+  }
+  v4d acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = v4d{0, 0, 0, 0};
+  double bet[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) bet[b] = 0;
+  double chi = 0;
+  // one group's loads: the rows of this lane's four points, and the model
+  // value and data of its own point 4q + c%4 (the convexity rows and the
+  // padding past them read the last real point's data, replaced at use)
+  struct Grp {
+    v4d raw[NB];
+    double ymq, sgq, wq, yoq;
+  };
+  auto load = [&](int g, Grp &G, bool pts) __attribute__((always_inline)) {
+#pragma unroll
+    for (int b = 0; b < NB; b++) G.raw[b] = *reinterpret_cast<const v4d *>(J + roff[b] + 16 * g);
+    const int q = 16 * g + 4 * kq + i4, qc = min(q, npts - 1);
+    G.ymq = ym[q];
+    if (pts) {
+      const int pt = perm ? perm[qc] : qc;
+      G.sgq = P.sig[pt];
+      G.wq = P.w[pt];
+      G.yoq = P.y[pt];
+    }
+  };
+  for (int lc = 0; lc <= P.n_lc; lc++) {
+    const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
+    const bool inrel = P.lc_inrel[lc] != 0;
+    const int g0 = q0 >> 4, g1 = (q1 + 15) >> 4;
+    double ave = 1;
+    double dr[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) dr[b] = 0;
+    if (inrel) {
+      // pass 1: ave = sum ymod, dave = sum dyda over the light curve
+      double sy = 0, sd[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++) sd[b] = 0;
+      Grp cur, nxt;
+      int g = g0 + wave;
+      if (g < g1) load(g, nxt, false);
+      for (; g < g1; g += kWaves) {
+        cur = nxt;
+        if (g + kWaves < g1) load(g + kWaves, nxt, false);
+        const int pb = 16 * g + 4 * kq;
+        {
+          const int q = pb + i4;
+          sy = sy + ((c16 < 4 && q >= q0 && q < q1) ? cur.ymq : 0.0);
+        }
+        if (16 * g >= q0 && 16 * g + 16 <= q1) {  // wave-uniform: every point of the group in the curve
+#pragma unroll
+          for (int b = 0; b < NB; b++) sd[b] = sd[b] + ((cur.raw[b][0] + cur.raw[b][1]) + (cur.raw[b][2] + cur.raw[b][3]));
+        } else {
+          bool lv[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) lv[i] = pb + i >= q0 && pb + i < q1;
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            const double s = ((lv[0] ? cur.raw[b][0] : 0.0) + (lv[1] ? cur.raw[b][1] : 0.0)) +
+                             ((lv[2] ? cur.raw[b][2] : 0.0) + (lv[3] ? cur.raw[b][3] : 0.0));
+            sd[b] = sd[b] + s;
+          }
+        }
+      }
+      // lanes of one column, then waves, in a fixed order
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        sd[b] = sd[b] + __shfl_xor(sd[b], 16, 64);
+        sd[b] = sd[b] + __shfl_xor(sd[b], 32, 64);
+      }
+      // the model-value sum: lanes c < 4 of each row hold its points
+      sy = sy + __shfl_xor(sy, 1, 64);
+      sy = sy + __shfl_xor(sy, 2, 64);
+      sy = sy + __shfl_xor(sy, 16, 64);
+      sy = sy + __shfl_xor(sy, 32, 64);
+      if (kq == 0) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) csum[wave * (NC + 1) + 16 * b + c16] = sd[b];
+        if (c16 == 0) csum[wave * (NC + 1) + NC] = sy;
+      }
+      __syncthreads();
+      if (tid <= NC) {
+        double s = csum[tid];
+        for (int w = 1; w < kWaves; w++) s = s + csum[w * (NC + 1) + tid];
+        dave[tid] = s;
+      }
+      __syncthreads();
+      ave = dave[NC];
+      if (tid < NC) drat[tid] = dave[tid] / ave;
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB; b++) dr[b] = drat[16 * b + c16];
+    }
+    // pass 2: renormalised rows, normal equations, beta and chi2
+    Grp cur, nxt;
+    int g = g0 + wave;
+    if (g < g1) load(g, nxt, true);
+    for (; g < g1; g += kWaves) {
+      cur = nxt;
+      if (g + kWaves < g1) load(g + kWaves, nxt, true);
+      const int pb = 16 * g + 4 * kq;
+      const bool full = 16 * g >= q0 && 16 * g + 16 <= q1;  // wave-uniform
+      // this lane's own point: renormalisation factor, model value,
+      // residual, 1/sigma^2 and weight (zero weight outside the curve); chi2
+      double coef, ymr, s2, wg, dy;
+      {
+        const int q = pb + i4;
+        const bool live = q >= q0 && q < q1, real = q < npts;
+        const double sg = real ? cur.sgq : sconv;
+        wg = live ? (real ? cur.wq : 1.0) : 0.0;
+        const double yo = real ? cur.yoq : 0.0;
+        coef = inrel ? sg * np / ave : 1.0;
+        ymr = live ? cur.ymq : 0.0;
+        dy = live ? yo - (inrel ? coef * ymr : ymr) : 0.0;
+        s2 = 1 / (sg * sg);
+        if (c16 < 4) chi = chi + dy * dy * s2 * wg;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const double cf = inrel ? row_bcast_d(coef, i) : 1.0, ymi = inrel ? row_bcast_d(ymr, i) : 0.0;
+        const double s2i = row_bcast_d(s2, i), wgi = row_bcast_d(wg, i), dyi = row_bcast_d(dy, i);
+        const bool live = full || (pb + i >= q0 && pb + i < q1);
+        double av[NB], bv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          double jn = cur.raw[b][i];
+          if (inrel) jn = l0[b] ? 0.0 : cf * (jn - ymi * dr[b]);
+          if (!full) jn = live ? jn : 0.0;
+          const double wt = jn * s2i * wgi;
+          bet[b] = bet[b] + dyi * wt;
+          bv[b] = jn;
+          av[b] = wt;
+        }
+        static_for<0, NB>([&](auto jbc) __attribute__((always_inline)) {
+          constexpr int jb = decltype(jbc)::value;
+          static_for<0, jb + 1>([&](auto kbc) __attribute__((always_inline)) {
+            constexpr int kb = decltype(kbc)::value;
+            constexpr int t = jb * (jb + 1) / 2 + kb;
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jb], bv[kb], acc[t], 0, 0, 0);
+          });
+        });
+      }
+    }
+  }
+  // beta: lanes of a column, then the waves in order
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    bet[b] = bet[b] + __shfl_xor(bet[b], 16, 64);
+    bet[b] = bet[b] + __shfl_xor(bet[b], 32, 64);
+    if (kq == 0) bsum[wave * NC + 16 * b + c16] = bet[b];
+  }
+  // chi2: lanes c < 4 of each row hold its points' terms
+  chi = chi + __shfl_xor(chi, 1, 64);
+  chi = chi + __shfl_xor(chi, 2, 64);
+  chi = chi + __shfl_xor(chi, 16, 64);
+  chi = chi + __shfl_xor(chi, 32, 64);
+  if (lane == 0) csq[wave] = chi;
+  __syncthreads();
+  if (tid < mfit) {
+    double s = bsum[tid];
+    for (int w = 1; w < kWaves; w++) s = s + bsum[w * NC + tid];
+    beta[tid] = s;
+  }
+  if (tid == 0) *chi_out = ((csq[0] + csq[1]) + csq[2]) + csq[3];
+  // alpha: four tiles at a time through LDS, each tile summed over the
+  // waves in order by one wave; j >= k entries stored and mirrored.
+  // FP64 D of 16x16x4: D[4r + l/16][l%16]
+  static_for<0, (NT + 3) / 4>([&](auto rc) __attribute__((always_inline)) {
+    constexpr int R = decltype(rc)::value;
+    static_for<0, 4>([&](auto uc) __attribute__((always_inline)) {
+      constexpr int u = decltype(uc)::value, t = 4 * R + u;
+      if constexpr (t < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) tred[((u * kWaves + wave) * 4 + r) * 64 + lane] = acc[t][r];
+      }
+    });
+    __syncthreads();
+    const int t = 4 * R + wave;
+    if (t < NT) {
+      int jb = 0;
+      while ((jb + 1) * (jb + 2) / 2 <= t) jb++;
+      const int kb = t - jb * (jb + 1) / 2;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double s = tred[((wave * kWaves + 0) * 4 + r) * 64 + lane];
+        for (int w = 1; w < kWaves; w++) s = s + tred[((wave * kWaves + w) * 4 + r) * 64 + lane];
+        const int j = 16 * jb + 4 * r + kq, k = 16 * kb + c16;
+        if (j < mfit && k < mfit && j >= k) {
+          alpha[j * mfit + k] = s;
+          alpha[k * mfit + j] = s;
+        }
+      }
+    }
+    __syncthreads();
+  });
+}
+
 // Scratch at the start of the normal-equation kernel's LDS: the convexity
 // rows' partial sums, then the staged rows (ks = 0) or the K-split
 // reductions (ks = column blocks).
 __host__ __device__ inline int neq_scratch_doubles(int mfit, bool mix, int ks) {
   const int d = neq_jn_doubles(mfit, mix);
-  return ks > 0 && neq_ks_doubles(ks) > d ? neq_ks_doubles(ks) : d;
+  const int k = ks > 0 ? (mix ? neq_ks_doubles(ks) : neq_ks64_doubles(ks)) : 0;
+  return k > d ? k : d;
 }
 
 // Fit initialisation (period_search_BOINC.cpp:819-844): shape from the
@@ -3051,11 +3297,14 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
     conv_rows<MIX>(P, mem + M.area, mem + M.g, mem + M.J, scal + 1, wst);
     __threadfence_block();
     __syncthreads();
-    if constexpr (KS > 0)
+    if constexpr (KS > 0 && MIX)
       // the points' brightness, sigma and weight/sigma^2 in the Jacobian's column order
       normal_equations_ks<KS, ABF>(P, mem + M.J, W.order ? mem + M.pys : P.y, W.order ? mem + M.psg : P.sig,
                               W.order ? mem + M.psw : P.swt, W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave,
                               drat, fidx, btr, scal, atr);
+    else if constexpr (KS > 0)
+      normal_equations_ks64<KS>(P, mem + M.J, W.order ? reinterpret_cast<const int *>(mem + M.perm) : nullptr,
+                                W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx, btr, scal, atr);
     else
       normal_equations_mfma<NT, MIX>(P, mem + M.J, W.order ? reinterpret_cast<const int *>(mem + M.perm) : nullptr,
                                      W.conw_rs ? W.conw_rs[S.prob] : W.conw_r, wst, dave, drat, fidx, btr, scal, atr);
@@ -3147,8 +3396,16 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
   (void)atry;
 }
 
+// Waves per SIMD of the normal-equation kernels: the staged kernel 1; the
+// split kernels: FP64 2 up to 3 column blocks (accumulators and two groups'
+// rows fit 256 registers), else 1; bf16a up to 4 blocks 3; otherwise
+// PS_NEQ_KS_WAVES.
+constexpr int neq_waves(bool mix, int ks, bool abf) {
+  return ks == 0 ? 1 : (!mix ? (ks <= 3 ? 2 : 1) : ((abf && ks <= 4) ? 3 : PS_NEQ_KS_WAVES));
+}
+
 template <int NT, bool MIX, bool MULTI, int KS = 0, bool ABF = false>
-__global__ void __launch_bounds__(kThreads, KS > 0 ? ((ABF && KS <= 4) ? 3 : PS_NEQ_KS_WAVES) : 1) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+__global__ void __launch_bounds__(kThreads, neq_waves(MIX, KS, ABF)) neq_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
   neq_body<NT, MIX, MULTI, KS, ABF>(P0, probs, W);
 }
 
@@ -3161,7 +3418,7 @@ __global__ void __launch_bounds__(kThreads, KS > 0 ? ((ABF && KS <= 4) ? 3 : PS_
 // solver, BR rows per wave, or the LDS solver, BR = 0).  An LM iteration is
 // then two launches: this kernel and the model kernel.
 template <int NT, bool MIX, bool MULTI, int KS, int BR, bool ABF = false>
-__global__ void __launch_bounds__(kThreads, 2) iter_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
+__global__ void __launch_bounds__(kThreads, (!MIX && KS > 4) ? 1 : 2) iter_kernel(DevProblem P0, const DevProblem *__restrict__ probs, DevWork W) {
   neq_body<NT, MIX, MULTI, KS, ABF>(P0, probs, W);
   __syncthreads();
   step_body<MULTI, 0, BR>(P0, probs, W);
@@ -3417,10 +3674,10 @@ StepFn pick_step(int mfit, bool diag) {
 
 using NeqFn = void (*)(DevProblem, const DevProblem *, DevWork);
 // Column blocks of the K-split normal equations for mfit free parameters
-// (mixed forms), or 0: the staged kernel.
-inline int neq_ks_blocks(int mfit, bool mix, bool staged) {
+// (every form), or 0: the staged kernel.
+inline int neq_ks_blocks(int mfit, bool staged) {
   const int nb = (mfit + 15) / 16;
-  return mix && !staged && nb <= 6 ? nb : 0;
+  return !staged && nb <= 6 ? nb : 0;
 }
 
 template <bool MIX, bool MULTI>
@@ -3435,15 +3692,15 @@ NeqFn pick_neq(int mfit, int ks, bool abf) {
         case 6: return neq_kernel<1, true, MULTI, 6, true>;
         default: break;
       }
-    switch (ks) {
-      case 1: return neq_kernel<1, true, MULTI, 1>;
-      case 2: return neq_kernel<1, true, MULTI, 2>;
-      case 3: return neq_kernel<1, true, MULTI, 3>;
-      case 4: return neq_kernel<1, true, MULTI, 4>;
-      case 5: return neq_kernel<1, true, MULTI, 5>;
-      case 6: return neq_kernel<1, true, MULTI, 6>;
-      default: break;
-    }
+  }
+  switch (ks) {
+    case 1: return neq_kernel<1, MIX, MULTI, 1>;
+    case 2: return neq_kernel<1, MIX, MULTI, 2>;
+    case 3: return neq_kernel<1, MIX, MULTI, 3>;
+    case 4: return neq_kernel<1, MIX, MULTI, 4>;
+    case 5: return neq_kernel<1, MIX, MULTI, 5>;
+    case 6: return neq_kernel<1, MIX, MULTI, 6>;
+    default: break;
   }
   const int nb = (mfit + 15) / 16, ntile = nb * (nb + 1) / 2, nt = (ntile + 3) / 4;
   if (nt <= 1) return neq_kernel<1, MIX, MULTI>;
@@ -3461,9 +3718,9 @@ NeqFn pick_iter(int mfit, int ks, bool abf) {
   if (MIX && abf && ks == 4 && mfit <= 64) return iter_kernel<1, MIX, MULTI, 4, 16, true>;
   if (MIX && abf && ks == 5) return iter_kernel<1, MIX, MULTI, 5, 24, true>;
   if (MIX && abf && ks == 6) return iter_kernel<1, MIX, MULTI, 6, 24, true>;
-  if (MIX && ks == 4 && mfit <= 64) return iter_kernel<1, MIX, MULTI, 4, 16>;
-  if (MIX && ks == 5) return iter_kernel<1, MIX, MULTI, 5, 24>;
-  if (MIX && ks == 6) return iter_kernel<1, MIX, MULTI, 6, 24>;
+  if (ks == 4 && mfit <= 64) return iter_kernel<1, MIX, MULTI, 4, 16>;
+  if (ks == 5) return iter_kernel<1, MIX, MULTI, 5, 24>;
+  if (ks == 6) return iter_kernel<1, MIX, MULTI, 6, 24>;
   if (ks == 0) {
     const int nb = (mfit + 15) / 16;
     if (nb == 4) return iter_kernel<3, MIX, MULTI, 0, 16>;
@@ -3644,7 +3901,7 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   const int fpad = (F + kFacetChunk - 1) / kFacetChunk * kFacetChunk;
   ks.lds_model = mixed ? (e->mixed_shared || mode >= 2 ? mixed_layout(F, nb, mode).total : mixedw_layout(fpad, nb, mode).total)
                        : (!ks.use_mfma ? model_lds(F) : (e->f64_barrier ? mfma_lds(F, ncoef) : f64_lds(F, ncoef)));
-  const int neq_blocks = neq_ks_blocks(mfit, mixed, e->neq_staged);
+  const int neq_blocks = neq_ks_blocks(mfit, e->neq_staged);
   // ordered points: the mixed forms with the split normal equations, and
   // FP64 with the overlapped model kernel
   ks.order = e->order_points && (mixed ? neq_blocks > 0 : ks.use_mfma && !e->f64_barrier) ? 1 : 0;
@@ -3660,11 +3917,11 @@ int build_kernels(ps_engine *e, const DevProblem &G, int nb, bool multi, bool al
   ks.step_threads = wave_solve ? 64 : kThreads;
   if (e->fuse) {
     ks.iter = mixed ? (multi ? pick_iter<true, true>(mfit, neq_blocks, abf) : pick_iter<true, false>(mfit, neq_blocks, abf))
-                    : (multi ? pick_iter<false, true>(mfit, 0, false) : pick_iter<false, false>(mfit, 0, false));
+                    : (multi ? pick_iter<false, true>(mfit, neq_blocks, false) : pick_iter<false, false>(mfit, neq_blocks, false));
     ks.lds_iter = std::max(ks.lds_step, ks.lds_neq);
   }
   ks.neq = mixed ? (multi ? pick_neq<true, true>(mfit, neq_blocks, abf) : pick_neq<true, false>(mfit, neq_blocks, abf))
-                 : (multi ? pick_neq<false, true>(mfit, 0, false) : pick_neq<false, false>(mfit, 0, false));
+                 : (multi ? pick_neq<false, true>(mfit, neq_blocks, false) : pick_neq<false, false>(mfit, neq_blocks, false));
   if (!ks.neq) {
     e->err = "too many free parameters";
     return PS_ERR_LIMIT;
