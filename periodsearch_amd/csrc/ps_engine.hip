@@ -3397,11 +3397,11 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
 }
 
 // Waves per SIMD of the normal-equation kernels: the staged kernel 1; the
-// split kernels: FP64 2 up to 3 column blocks (accumulators and two groups'
-// rows fit 256 registers), else 1; bf16a up to 4 blocks 3; otherwise
-// PS_NEQ_KS_WAVES.
+// split kernels: FP64 2 up to 4 column blocks (at 4, 12 registers spill:
+// measured faster than one wave per SIMD), else 1; bf16a up to 4 blocks 3;
+// otherwise PS_NEQ_KS_WAVES.
 constexpr int neq_waves(bool mix, int ks, bool abf) {
-  return ks == 0 ? 1 : (!mix ? (ks <= 3 ? 2 : 1) : ((abf && ks <= 4) ? 3 : PS_NEQ_KS_WAVES));
+  return ks == 0 ? 1 : (!mix ? (ks <= 4 ? 2 : 1) : ((abf && ks <= 4) ? 3 : PS_NEQ_KS_WAVES));
 }
 
 template <int NT, bool MIX, bool MULTI, int KS = 0, bool ABF = false>
