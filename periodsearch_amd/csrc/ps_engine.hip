@@ -883,6 +883,7 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
   for (int lc = 0; lc <= P.n_lc; lc++) {
     const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
     const bool inrel = P.lc_inrel[lc] != 0;
+    const bool z0 = inrel && l0[0];  // this lane's block-0 column is the size coefficient, zeroed here
     const int g0 = q0 >> 4, g1 = (q1 + 15) >> 4;
     double rave = 0;  // np / ave
     float dr[NB];
@@ -973,7 +974,7 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
         const double dy = live ? yo - cy : 0.0;
         if (c16 < 4) chi = chi + dy * dy * sw;
         coef_f = (float)coef;
-        cy_f = (float)cy;
+        cy_f = inrel ? (float)cy : 0.0f;  // zero in absolute curves: the renormalisation below returns the row
         sw_f = (float)sw;
         swdy_f = (float)(sw * dy);
       }
@@ -993,7 +994,12 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           float jn = cur.raw[b][i];
-          if (inrel) jn = l0[b] ? 0.0f : cf * jn - cyi * dr[b];
+          // coef 1, coef*ymod 0 and dave/ave 0 in absolute curves: 1*jn - 0*0 is jn exactly
+          jn = cf * jn - cyi * dr[b];
+          // the size coefficient, zeroed in relative curves, can only be free
+          // parameter 0 (free parameters in increasing order; the padding
+          // columns past mfit are discarded)
+          if (b == 0) jn = z0 ? 0.0f : jn;
           if (!full) jn = live ? jn : 0.0f;
           bp[b] = bp[b] + jn * swdyi;
           bv[b] = jn;
@@ -1167,6 +1173,7 @@ __device__ __forceinline__ void normal_equations_ks64(const DevProblem &P, const
   for (int lc = 0; lc <= P.n_lc; lc++) {
     const int q0 = P.lc_start[lc], q1 = P.lc_start[lc + 1], np = q1 - q0;
     const bool inrel = P.lc_inrel[lc] != 0;
+    const bool z0 = inrel && l0[0];  // this lane's block-0 column is the size coefficient, zeroed here
     const int g0 = q0 >> 4, g1 = (q1 + 15) >> 4;
     double ave = 1;
     double dr[NB];
@@ -1258,14 +1265,19 @@ __device__ __forceinline__ void normal_equations_ks64(const DevProblem &P, const
       }
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const double cf = inrel ? row_bcast_d(coef, i) : 1.0, ymi = inrel ? row_bcast_d(ymr, i) : 0.0;
+        const double cf = row_bcast_d(coef, i), ymi = row_bcast_d(inrel ? ymr : 0.0, i);
         const double s2i = row_bcast_d(s2, i), wgi = row_bcast_d(wg, i), dyi = row_bcast_d(dy, i);
         const bool live = full || (pb + i >= q0 && pb + i < q1);
         double av[NB], bv[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           double jn = cur.raw[b][i];
-          if (inrel) jn = l0[b] ? 0.0 : cf * (jn - ymi * dr[b]);
+          // coef 1, ymod 0 and dave/ave 0 in absolute curves: 1*(jn - 0*0) is jn exactly
+          jn = cf * (jn - ymi * dr[b]);
+          // the size coefficient, zeroed in relative curves, can only be free
+          // parameter 0 (free parameters in increasing order; the padding
+          // columns past mfit are discarded)
+          if (b == 0) jn = z0 ? 0.0 : jn;
           if (!full) jn = live ? jn : 0.0;
           const double wt = jn * s2i * wgi;
           bet[b] = bet[b] + dyi * wt;
