@@ -255,11 +255,19 @@ def samples_for(eng: Engine, cfg: dict, tmpdir: str, n_near: int = 4):
     npol = inp.problem.n_poles
     x = features(eng, inp, freq)
     t = np.zeros((len(n), N_OUT))
+    keep = np.zeros(len(n), dtype=bool)
     for i in range(len(n)):
         f = fits[i * npol:(i + 1) * npol]
-        b = int(np.argmin(f["dev"]))
+        # the first strictly smaller dev wins and a NaN never does
+        # (period_search_BOINC.cpp:913-924); a period whose pole fits all
+        # diverged gives no target
+        dev = f["dev"]
+        if not np.isfinite(dev).any():
+            continue
+        b = int(np.nanargmin(np.where(np.isfinite(dev), dev, np.nan)))
         t[i] = encode_target(params[i * npol + b], cfg["lmax"], cfg["mmax"])
-    return x, t, len(fits)
+        keep[i] = True
+    return x[keep], t[keep], len(fits)
 
 
 def make_training_set(n_asteroids: int, seed0: int = 1_000_000, log=print):
