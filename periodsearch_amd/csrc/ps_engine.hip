@@ -994,14 +994,16 @@ __device__ __forceinline__ void normal_equations_ks(const DevProblem &P, const d
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           float jn = cur.raw[b][i];
-          // coef 1, coef*ymod 0 and dave/ave 0 in absolute curves: 1*jn - 0*0 is jn exactly
-          jn = cf * jn - cyi * dr[b];
+          // coef*dyda - (coef*ymod)*(dave/ave), the first product fused; coef
+          // 1, coef*ymod 0 and dave/ave 0 in absolute curves: jn + (-0) is jn
+          // exactly
+          jn = fmaf(cf, jn, -(cyi * dr[b]));
           // the size coefficient, zeroed in relative curves, can only be free
           // parameter 0 (free parameters in increasing order; the padding
           // columns past mfit are discarded)
           if (b == 0) jn = z0 ? 0.0f : jn;
           if (!full) jn = live ? jn : 0.0f;
-          bp[b] = bp[b] + jn * swdyi;
+          bp[b] = fmaf(jn, swdyi, bp[b]);
           bv[b] = jn;
           av[b] = jn * swi;
           if constexpr (ABF) {
@@ -3672,6 +3674,7 @@ StepFn pick_step(int mfit, bool diag) {
   if (mfit <= 0 || mfit > kBlockSolveMax) return step_kernel<MULTI, 0>;
   if (mfit > kWaveSolveMax) {
     if (mfit <= 64) return diag ? step_kernel<MULTI, 0, 16, true> : step_kernel<MULTI, 0, 16>;
+    if (mfit <= 88) return diag ? step_kernel<MULTI, 0, 22, true> : step_kernel<MULTI, 0, 22>;
     if (mfit <= 96) return diag ? step_kernel<MULTI, 0, 24, true> : step_kernel<MULTI, 0, 24>;
     return diag ? step_kernel<MULTI, 0, 32, true> : step_kernel<MULTI, 0, 32>;
   }
