@@ -3411,7 +3411,7 @@ __device__ __forceinline__ void neq_body(const DevProblem &P0, const DevProblem 
 }
 
 // Waves per SIMD of the normal-equation kernels: the staged kernel 1; the
-// split kernels: FP64 2 up to 4 column blocks (at 4, 12 registers spill:
+// split kernels: FP64 2 up to 4 column blocks (at 4, 25 registers spill:
 // measured faster than one wave per SIMD), else 1; bf16a up to 4 blocks 3;
 // otherwise PS_NEQ_KS_WAVES.
 constexpr int neq_waves(bool mix, int ks, bool abf) {
